@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the plain GPU product path: product limbs/s at 2^22 x 2^22.
+
+One step = one bit-exact 2^22 x 2^22-limb product (preclib's mul_ntt path:
+digits, forward NTTs, pointwise product, inverse NTT, CRT, carry) with both
+operands already resident in HBM and the product left in HBM.  Operands are
+synthetic (splitmix64 uniform limbs, top limb nonzero).
+
+Prints one JSON line.  Besides the contract fields it carries
+  roofline     -- the dominant kernel: algorithmic bytes per launch (its share
+                  of the byte model B = 16(Na+Nb) + 48 P n) over its average
+                  launch time from HIP events on the library's stream;
+  cpu_baseline -- the reference's own compiled product (oracle/_ref, built
+                  from the reference sources) on the same operands, 1 core;
+  product_roofline -- the whole product against the same byte model.
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X published HBM3E bandwidth, bytes/s
+NPRIMES = 2
+
+
+def model_bytes(an, bn, n):
+    """Per-kernel share of the byte model ('Byte model')."""
+    P = NPRIMES
+    return {
+        "ntt_col_fwd": 8.0 * (an + bn) + 16.0 * P * n,
+        "ntt_row": 24.0 * P * n,
+        "ntt_col_inv": 8.0 * P * n,
+        "carry": 8.0 * (an + bn),
+    }
+
+
+def transform_len(a, b):
+    def nd(top):
+        d = 0
+        top = int(top)
+        while top:
+            d += 1
+            top >>= 16
+        return d
+    da = 4 * (len(a) - 1) + nd(a[-1])
+    db = 4 * (len(b) - 1) + nd(b[-1])
+    n = 1
+    while n < da + db:
+        n <<= 1
+    return n
+
+
+def cpu_baseline(a, b, budget_s=25.0):
+    """The reference's mul_into on the same operands, one core (bounded)."""
+    import ctypes
+    ref = os.path.join(ROOT, "oracle", "_ref", "libpreclib_ref.so")
+    P64 = ctypes.POINTER(ctypes.c_uint64)
+    r = np.zeros(len(a) + len(b), np.uint64)
+    if os.path.exists(ref):
+        L = ctypes.CDLL(ref)
+        f = L.ref_mul_into
+        f.restype = ctypes.c_size_t
+        f.argtypes = [P64, ctypes.c_size_t, P64, ctypes.c_size_t, P64, ctypes.c_size_t, ctypes.c_int]
+        kind = "reference"
+        call = lambda: f(r.ctypes.data_as(P64), len(r), a.ctypes.data_as(P64), len(a),
+                         b.ctypes.data_as(P64), len(b), 0)
+    else:
+        from tests.oracle_py import Oracle
+        o = Oracle()
+        kind = "port"
+        call = lambda: o.mul_into(a, b)
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        call()
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start + times[-1] > budget_s or len(times) >= 3:
+            break
+    med = sorted(times)[len(times) // 2]
+    return {"value": (len(a) + len(b)) / med, "unit": "product limbs/s", "cores": 1, "kind": kind,
+            "sample": "%d full %dx%d-limb products (median %.2f s), pinned to one host core"
+                      % (len(times), len(a), len(b), med)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--logn", type=int, default=22, help="operand size 2^logn limbs")
+    ap.add_argument("--dist", default="uniform")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the product path "
+              "shards as described in DESIGN.md, but no multi-GPU run is part of this build)" % args.gpus)
+        sys.exit(2)
+
+    from preclib_amd import pgpu, workloads
+    pgpu.init(0)
+    N = 1 << args.logn
+    a, b = workloads.pair(N, N, args.dist)
+    n = transform_len(a, b)
+    da, db = pgpu.DevBuf.of(a), pgpu.DevBuf.of(b)
+    r = pgpu.DevBuf(2 * N)
+    for _ in range(args.warmup):
+        pgpu.mul_dev(r, da, N, db, N, want_len=False)
+    pgpu.sync()
+    # correctness of the measured configuration (outside the timed region)
+    rl, _ = pgpu.mul_dev(r, da, N, db, N)
+    got = r.download(rl)
+    check = None
+    gold = os.path.join(ROOT, "tests", "golden", "products.json")
+    if os.path.exists(gold):
+        for e in json.load(open(gold))["workload"]:
+            if e["an"] == N and e["bn"] == N and e["dist"] == args.dist:
+                check = hashlib.sha256(got.astype("<u8").tobytes()).hexdigest() == e["sha256"]
+    if check is False:
+        print("bench.py: product differs from the golden digest", file=sys.stderr)
+        sys.exit(1)
+
+    pgpu.timing(True)
+    pgpu.timing_reset()
+    pgpu.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pgpu.mul_dev(r, da, N, db, N, want_len=False)
+    pgpu.sync()
+    t1 = time.perf_counter()
+    stages = pgpu.timing_read()
+    pgpu.timing(False)
+
+    dt = t1 - t0
+    ms = dt / args.steps * 1e3
+    value = args.steps * 2 * N / dt
+    mb = model_bytes(N, N, n)
+    dom = max(stages, key=lambda k: stages[k][1])
+    launches, tot_ms, _ = stages[dom]
+    avg_s = tot_ms / launches * 1e-3
+    achieved = mb[dom] / avg_s
+    prod_bytes = 16.0 * 2 * N + 48.0 * NPRIMES * n
+    line = {
+        "metric": "product limbs/s at 2^22 limbs (plain GPU NTT product, device-resident)",
+        "value": value,
+        "unit": "product limbs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 modular (NTT), u64 limbs",
+        "data": "synthetic (splitmix64 %s limbs, top limb nonzero)" % args.dist,
+        "config": {"workload": "single product %dx%d limbs, NTT n=2^%d, P=%d primes" %
+                   (N, N, n.bit_length() - 1, NPRIMES), "an": N, "bn": N, "dist": args.dist,
+                   "bit_exact_vs_golden": check},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "avg_launch_ms": avg_s * 1e3},
+        "product_roofline": {"bytes": prod_bytes, "achieved_GBps": prod_bytes / (ms * 1e-3) / 1e9,
+                             "frac": prod_bytes / (ms * 1e-3) / HBM_PEAK},
+        "stages_ms": {k: v[1] / v[0] for k, v in stages.items()},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(a, b)
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = value / cb["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,98 @@
+/*
+ * pgpu.h -- C ABI of the MI355X (gfx950) product library, libpgpu.so.
+ *
+ * This is the drop-in boundary for preclib's large-integer product path: the
+ * host precn_t layer calls these entry points at the branches where preclib's
+ * dispatcher calls its transform multipliers (src/mul_basic.cpp:259-269 and
+ * 203).  All buffers are plain little-endian uint64 limb arrays; every size
+ * is a limb count.  No entry point frees or reallocates caller memory.
+ *
+ * Status codes: 0 on success, a negative PG_E* value on failure; the text of
+ * the last failure on the calling thread is returned by pg_last_error().
+ * The library is internally serialized (one device context per process).
+ */
+#ifndef PGPU_H
+#define PGPU_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PG_OK = 0,
+    PG_EINVAL = -1,    /* bad argument (size, null pointer, capacity)        */
+    PG_ENOMEM = -2,    /* device memory budget or hipMalloc exhausted        */
+    PG_EDEVICE = -3,   /* HIP runtime error                                  */
+    PG_ERANGE = -4,    /* operand outside the path's supported size range    */
+    PG_ENOINIT = -5    /* pg_init not called                                 */
+};
+
+/* Product paths.  PG_PATH_AUTO applies preclib's dispatcher rules for sizes
+ * above 192 limbs (mul_basic.cpp:238-276): blocked when hi > 2*lo, NTT when
+ * hi > 6144, FFT otherwise.  PG_PATH_NTT / PG_PATH_FFT force the transform
+ * (mul_ntt / mul_fft semantics, including mul_ntt's fall back to the FFT
+ * above 2^24 limbs or 2^26 points, mul_ntt.cpp:497-506). */
+enum { PG_PATH_AUTO = 0, PG_PATH_NTT = 1, PG_PATH_FFT = 2 };
+
+/* Opens device `device` and sets the device-memory budget in bytes (0 = 90%
+ * of free HBM).  Replaces nothing in preclib (it has no device); must precede
+ * every other call.  Calling it again with the same device is a no-op. */
+int pg_init(int device, size_t hbm_budget);
+void pg_shutdown(void);
+const char *pg_last_error(void);
+
+/*
+ * r = a * b for host buffers.  Replaces mul_into's transform branches
+ * (src/mul_basic.cpp:259-269) and, with path NTT / FFT, the public
+ * precn_t mul_ntt(a, b) (src/mul_ntt.cpp:494-518) and
+ * precn_t mul_fft(a, b) (src/mul_fft.cpp:386-394).
+ * Requires an, bn >= 1, top limbs nonzero, max(an, bn) > 192 and
+ * r_cap >= an + bn.  r may alias a or b.  *r_len receives the trimmed length
+ * (an+bn-1 or an+bn).  If fft_max_round is not NULL it receives the largest
+ * |v - nearbyint(v)| over the FP64 coefficients of every FFT run by the call
+ * (0 when none ran), the quantity COUNT_FFTS reports (mul_fft.cpp:359-382).
+ */
+int pg_mul(uint64_t *r, size_t r_cap, size_t *r_len,
+           const uint64_t *a, size_t an, const uint64_t *b, size_t bn,
+           int path, double *fft_max_round);
+
+/* Device-resident operands.  A buffer is a limb array in HBM owned by the
+ * library's pool.  There is no preclib counterpart: preclib keeps every
+ * operand in malloc'd host memory (init_basic.cpp:47-135). */
+typedef struct pg_buf pg_buf;
+int pg_buf_alloc(pg_buf **h, size_t limbs);
+int pg_buf_free(pg_buf *h);
+size_t pg_buf_capacity(const pg_buf *h);
+int pg_buf_upload(pg_buf *h, const uint64_t *src, size_t limbs);
+int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs);
+/* Device pointer of the buffer's limbs (for callers that manage their own
+ * HIP work on the library's stream). */
+uint64_t *pg_buf_ptr(pg_buf *h);
+
+/* r = a * b with all three operands resident in HBM (same contract as
+ * pg_mul; r may alias a or b).  Asynchronous unless r_len is non-NULL, in
+ * which case the call waits to read the product length. */
+int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an,
+               const pg_buf *b, size_t bn, int path, double *fft_max_round);
+
+/* Waits for all queued device work. */
+int pg_sync(void);
+
+/* Per-stage kernel timing on the library's stream (HIP events around each
+ * launch).  pg_timing_read fills up to `cap` entries and returns the number of
+ * stages; names are stable identifiers such as "ntt_col_fwd". */
+typedef struct {
+    const char *name;
+    uint64_t launches;
+    double total_ms;
+    double bytes;      /* algorithmic bytes moved, summed over launches */
+} pg_stage_time;
+int pg_timing_enable(int on);
+int pg_timing_reset(void);
+int pg_timing_read(pg_stage_time *out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,196 @@
+// pg_api.hip -- extern "C" entry points of libpgpu.so (declared in pgpu.h).
+#include <algorithm>
+#include <cstring>
+#include "pg_runtime.h"
+#include "pgpu.h"
+
+using namespace pg;
+
+namespace pg {
+const char *last_error();
+int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,
+               size_t bn, uint64_t btop, int path, double *fft_max_round);
+}
+
+struct pg_buf {
+    uint64_t *d;
+    size_t cap;
+};
+
+#define LOCK std::lock_guard<std::recursive_mutex> lk_(ctx().mu)
+#define NEED_INIT \
+    if (!ctx().stream) return set_error(PG_ENOINIT, "pg_init not called")
+
+extern "C" {
+
+int pg_init(int device, size_t hbm_budget) { return ctx().init(device, hbm_budget); }
+
+void pg_shutdown(void) { ctx().shutdown(); }
+
+const char *pg_last_error(void) { return pg::last_error(); }
+
+int pg_sync(void) {
+    LOCK;
+    NEED_INIT;
+    if (hipStreamSynchronize(ctx().stream) != hipSuccess) return set_error(PG_EDEVICE, "stream sync failed");
+    return PG_OK;
+}
+
+int pg_buf_alloc(pg_buf **h, size_t limbs) {
+    LOCK;
+    NEED_INIT;
+    if (!h || !limbs) return set_error(PG_EINVAL, "pg_buf_alloc: bad argument");
+    void *p = ctx().pool.get(limbs * sizeof(uint64_t));
+    if (!p) return set_error(PG_ENOMEM, "pg_buf_alloc: device budget exhausted");
+    *h = new pg_buf{(uint64_t *)p, limbs};
+    return PG_OK;
+}
+
+int pg_buf_free(pg_buf *h) {
+    LOCK;
+    if (!h) return PG_OK;
+    ctx().pool.put(h->d);
+    delete h;
+    return PG_OK;
+}
+
+size_t pg_buf_capacity(const pg_buf *h) { return h ? h->cap : 0; }
+uint64_t *pg_buf_ptr(pg_buf *h) { return h ? h->d : nullptr; }
+
+int pg_buf_upload(pg_buf *h, const uint64_t *src, size_t limbs) {
+    LOCK;
+    NEED_INIT;
+    if (!h || (!src && limbs) || limbs > h->cap) return set_error(PG_EINVAL, "pg_buf_upload: bad argument");
+    if (hipMemcpyAsync(h->d, src, limbs * 8, hipMemcpyHostToDevice, ctx().stream) != hipSuccess ||
+        hipStreamSynchronize(ctx().stream) != hipSuccess)
+        return set_error(PG_EDEVICE, "pg_buf_upload: copy failed");
+    return PG_OK;
+}
+
+int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs) {
+    LOCK;
+    NEED_INIT;
+    if (!h || (!dst && limbs) || limbs > h->cap) return set_error(PG_EINVAL, "pg_buf_download: bad argument");
+    if (hipMemcpyAsync(dst, h->d, limbs * 8, hipMemcpyDeviceToHost, ctx().stream) != hipSuccess ||
+        hipStreamSynchronize(ctx().stream) != hipSuccess)
+        return set_error(PG_EDEVICE, "pg_buf_download: copy failed");
+    return PG_OK;
+}
+
+static int check_operands(size_t an, size_t bn, size_t r_cap) {
+    if (!an || !bn) return set_error(PG_EINVAL, "operands must have at least one limb");
+    if (r_cap < an + bn) return set_error(PG_EINVAL, "r_cap < an + bn");
+    if (std::max(an, bn) <= 192) return set_error(PG_ERANGE, "operands of <= 192 limbs take the host basecase");
+    return PG_OK;
+}
+
+int pg_mul(uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a, size_t an, const uint64_t *b,
+           size_t bn, int path, double *fft_max_round) {
+    LOCK;
+    NEED_INIT;
+    if (!r || !a || !b) return set_error(PG_EINVAL, "pg_mul: null pointer");
+    int st = check_operands(an, bn, r_cap);
+    if (st) return st;
+    if (!a[an - 1] || !b[bn - 1]) return set_error(PG_EINVAL, "pg_mul: operands must be trimmed");
+    Context &C = ctx();
+    uint64_t *da = (uint64_t *)C.pool.get(an * 8);
+    uint64_t *db = (uint64_t *)C.pool.get(bn * 8);
+    uint64_t *dr = (uint64_t *)C.pool.get((an + bn) * 8);
+    if (!da || !db || !dr) {
+        C.pool.put(da); C.pool.put(db); C.pool.put(dr);
+        return set_error(PG_ENOMEM, "pg_mul: device budget exhausted");
+    }
+    st = PG_OK;
+    if (hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess ||
+        hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess)
+        st = set_error(PG_EDEVICE, "pg_mul: upload failed");
+    if (!st) st = mul_device(dr, da, an, a[an - 1], db, bn, b[bn - 1], path, fft_max_round);
+    if (!st && hipMemcpyAsync(r, dr, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess)
+        st = set_error(PG_EDEVICE, "pg_mul: download failed");
+    if (hipStreamSynchronize(C.stream) != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul: device error");
+    C.pool.put(da); C.pool.put(db); C.pool.put(dr);
+    if (st) return st;
+    size_t len = an + bn;
+    while (len && r[len - 1] == 0) --len;
+    if (r_len) *r_len = len;
+    return PG_OK;
+}
+
+int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an, const pg_buf *b, size_t bn, int path,
+               double *fft_max_round) {
+    LOCK;
+    NEED_INIT;
+    if (!r || !a || !b) return set_error(PG_EINVAL, "pg_mul_dev: null handle");
+    if (an > a->cap || bn > b->cap) return set_error(PG_EINVAL, "pg_mul_dev: length beyond buffer");
+    int st = check_operands(an, bn, r->cap);
+    if (st) return st;
+    Context &C = ctx();
+    uint64_t tops[2];
+    if (hipMemcpyAsync(&tops[0], a->d + an - 1, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
+        hipMemcpyAsync(&tops[1], b->d + bn - 1, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
+        hipStreamSynchronize(C.stream) != hipSuccess)
+        return set_error(PG_EDEVICE, "pg_mul_dev: reading top limbs failed");
+    st = mul_device(r->d, a->d, an, tops[0], b->d, bn, tops[1], path, fft_max_round);
+    if (st) return st;
+    if (r_len) {
+        uint64_t top = 0;
+        if (hipMemcpyAsync(&top, r->d + an + bn - 1, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
+            hipStreamSynchronize(C.stream) != hipSuccess)
+            return set_error(PG_EDEVICE, "pg_mul_dev: device error");
+        *r_len = top ? an + bn : an + bn - 1;
+    }
+    return PG_OK;
+}
+
+int pg_timing_enable(int on) {
+    LOCK;
+    ctx().timing = on != 0;
+    return PG_OK;
+}
+
+int pg_timing_reset(void) {
+    LOCK;
+    if (ctx().stream) hipStreamSynchronize(ctx().stream);
+    ctx().collect_timing();
+    ctx().stats.clear();
+    return PG_OK;
+}
+
+int pg_timing_read(pg_stage_time *out, int cap) {
+    LOCK;
+    if (ctx().stream) hipStreamSynchronize(ctx().stream);
+    ctx().collect_timing();
+    int i = 0;
+    for (auto &kv : ctx().stats) {
+        if (out && i < cap) {
+            out[i].name = kv.first.c_str();
+            out[i].launches = kv.second.launches;
+            out[i].total_ms = kv.second.ms;
+            out[i].bytes = kv.second.bytes;
+        }
+        ++i;
+    }
+    return i;
+}
+
+}  // extern "C"
+
+namespace pg {
+// Size-based routing above 192 limbs, following mul_into (src/mul_basic.cpp:
+// 259-269) and mul_ntt's fallbacks (src/mul_ntt.cpp:497-506).
+int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp, size_t bn,
+               uint64_t btop, int path, double *fft_max_round) {
+    if (fft_max_round) *fft_max_round = 0.0;
+    const size_t lo = std::min(an, bn), hi = std::max(an, bn);
+    if (path == PG_PATH_AUTO) {
+        if (hi > 2 * lo) return set_error(PG_ERANGE, "blocked path not available yet");
+        path = hi > 6144 ? PG_PATH_NTT : PG_PATH_FFT;
+    }
+    if (path == PG_PATH_NTT) {
+        int st = ntt_mul_device(rp, ap, an, atop, bp, bn, btop);
+        if (st != PG_ERANGE) return st;
+        path = PG_PATH_FFT;   // beyond the NTT caps the reference multiplies with mul_fft
+    }
+    return set_error(PG_ERANGE, "FFT path not available yet");
+}
+}  // namespace pg

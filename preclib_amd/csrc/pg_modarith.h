@@ -1,0 +1,107 @@
+// pg_modarith.h -- 32-bit modular arithmetic for the NTT primes on gfx950.
+//
+// The primes are preclib's NTT set (src/mul_ntt.cpp:29-31), all below 2^31,
+// so a sum of two residues fits in 32 bits and every reduction below is a
+// single unsigned min().  Two multiplication forms are used:
+//  * Shoup's precomputed-quotient product x*w mod p for fixed twiddles
+//    (wq = floor(w * 2^32 / p)): 1 mul_hi + 2 mul_lo, result in [0, 2p),
+//    then one min() to [0, p).
+//  * Montgomery's product a*b*2^-32 mod p (the reference's mont_reduce,
+//    mul_ntt.cpp:43-48) for the pointwise spectrum product and for twiddles
+//    formed on the fly, which have no precomputed quotient.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pg {
+
+struct PrimeK {
+    uint32_t p;      // modulus
+    uint32_t ninv;   // -p^-1 mod 2^32
+    uint32_t w32[16], w32q[16];     // omega_32^k, Shoup quotients (forward)
+    uint32_t iw32[16], iw32q[16];   // omega_32^-k, Shoup quotients (inverse)
+};
+
+__device__ __forceinline__ uint32_t mod_add(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t s = a + b;
+    return min(s, s - p);
+}
+
+__device__ __forceinline__ uint32_t mod_sub(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t d = a - b;
+    return min(d, d + p);
+}
+
+// x * w mod p with wq = floor(w * 2^32 / p); x < 2^32, w < p.
+__device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint32_t w, uint32_t wq, uint32_t p) {
+    uint32_t q = __umulhi(x, wq);
+    uint32_t r = x * w - q * p;
+    return min(r, r - p);
+}
+
+// a * b * 2^-32 mod p for a * b < p * 2^32.
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t ninv) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * ninv;
+    uint64_t u = (t + (uint64_t)m * p) >> 32;
+    uint32_t r = (uint32_t)u;
+    return min(r, r - p);
+}
+
+template <int R>
+__device__ __forceinline__ constexpr int bitrev(int q) {
+    int r = 0;
+    for (int b = 1, c = R >> 1; b < R; b <<= 1, c >>= 1)
+        if (q & b) r |= c;
+    return r;
+}
+
+// R-point forward DFT in registers, Gentleman-Sande DIF: natural order in,
+// x[q] = X[bitrev(q)] out.  Twiddles omega_R^k = omega_32^(k * 32 / R).
+template <int R>
+__device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
+    const uint32_t p = c.p;
+#pragma unroll
+    for (int h = R / 2; h >= 1; h >>= 1) {
+#pragma unroll
+        for (int b = 0; b < R; b += 2 * h) {
+#pragma unroll
+            for (int j = 0; j < h; ++j) {
+                uint32_t u = x[b + j], v = x[b + j + h];
+                x[b + j] = mod_add(u, v, p);
+                uint32_t d = mod_sub(u, v, p);
+                if (j == 0) {
+                    x[b + j + h] = d;
+                } else {
+                    const int k = j * (16 / h);   // omega_{2h}^j = omega_32^(j*16/h)
+                    x[b + j + h] = mul_shoup(d, c.w32[k], c.w32q[k], p);
+                }
+            }
+        }
+    }
+}
+
+// R-point inverse DFT (unnormalized), Cooley-Tukey DIT: x[q] = Y[bitrev(q)]
+// in, natural order out.
+template <int R>
+__device__ __forceinline__ void dft_dit_inv(uint32_t (&x)[R], const PrimeK &c) {
+    const uint32_t p = c.p;
+#pragma unroll
+    for (int h = 1; h <= R / 2; h <<= 1) {
+#pragma unroll
+        for (int b = 0; b < R; b += 2 * h) {
+#pragma unroll
+            for (int j = 0; j < h; ++j) {
+                uint32_t u = x[b + j], v = x[b + j + h];
+                if (j != 0) {
+                    const int k = j * (16 / h);
+                    v = mul_shoup(v, c.iw32[k], c.iw32q[k], p);
+                }
+                x[b + j] = mod_add(u, v, p);
+                x[b + j + h] = mod_sub(u, v, p);
+            }
+        }
+    }
+}
+
+}  // namespace pg

@@ -1,0 +1,113 @@
+// pg_runtime.h -- host-side device context of libpgpu.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "pg_kernels.h"
+
+namespace pg {
+
+// NTT primes and generators (src/mul_ntt.cpp:29-34).
+constexpr uint32_t PRIME[3] = {469762049u, 1811939329u, 2013265921u};
+constexpr uint32_t GEN[3] = {3u, 13u, 31u};
+
+uint32_t powmod(uint32_t b, uint64_t e, uint32_t p);
+uint32_t shoup_q(uint32_t w, uint32_t p);
+uint32_t to_mont(uint32_t x, uint32_t p);
+PrimeK make_prime(int idx);
+
+struct Status {
+    int code;
+    std::string msg;
+};
+
+// Caching device allocator with a byte budget.  Blocks are kept in free
+// lists keyed by their rounded size; nothing is returned to HIP until
+// shutdown, so steady-state products never call hipMalloc.
+class Pool {
+public:
+    void set_budget(size_t b) { budget_ = b; }
+    void set_stream(hipStream_t s) { stream_ = s; }
+    void *get(size_t bytes);
+    void put(void *p);
+    void release_all();
+    size_t in_use() const { return in_use_; }
+    size_t reserved() const { return reserved_; }
+    size_t budget() const { return budget_; }
+
+private:
+    static size_t round(size_t b);
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> live_;
+    size_t budget_ = 0, in_use_ = 0, reserved_ = 0;
+    hipStream_t stream_ = nullptr;
+};
+
+// Device twiddle tables, built once per (prime, block size, radix) and per
+// (prime, n) for the four-step tables.
+struct TwTables {
+    const uint2 *fwd = nullptr, *inv = nullptr;
+};
+struct GlobalTw {
+    const uint32_t *glo = nullptr, *ghi = nullptr, *ilo = nullptr, *ihi = nullptr;
+    int lobits = 0;
+};
+
+struct NttPlan {
+    int logn = 0, logN2 = 0, logT = 0;
+    SubPlan col{}, row{};
+};
+
+struct StageStat {
+    uint64_t launches = 0;
+    double ms = 0.0, bytes = 0.0;
+};
+
+class Context {
+public:
+    int device = -1;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    std::recursive_mutex mu;
+    PrimeK pk[3];
+    uint32_t crt_inv = 0, crt_invq = 0;
+
+    int init(int dev, size_t budget);
+    void shutdown();
+
+    const TwTables &tw(int prime, int logB, int lr);
+    const GlobalTw &gtw(int prime, int logn);
+    static NttPlan ntt_plan(int logn);
+
+    // timing
+    bool timing = false;
+    std::map<std::string, StageStat> stats;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    void stage_begin(hipEvent_t &a);
+    void stage_end(const char *name, hipEvent_t a, double bytes);
+    void collect_timing();
+
+private:
+    std::map<long, TwTables> tw_;
+    std::map<long, GlobalTw> gtw_;
+    std::vector<void *> tables_;   // table allocations (freed at shutdown)
+    void *table_alloc(size_t bytes);
+};
+
+Context &ctx();
+int set_error(int code, const std::string &msg);
+
+// Product entry points on device pointers (stream-ordered on ctx().stream).
+// rp must hold an + bn limbs; it may alias ap or bp.  Returns a status code.
+int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
+                   const uint64_t *bp, size_t bn, uint64_t btop);
+
+}  // namespace pg

@@ -1,0 +1,424 @@
+// pg_runtime.hip -- device context, memory pool, plans and twiddle tables of
+// libpgpu.so, plus the NTT product driver.
+#include "pg_runtime.h"
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include "pgpu.h"
+
+namespace pg {
+
+static thread_local std::string g_err;
+int set_error(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+const char *last_error() { return g_err.c_str(); }
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_error(PG_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {
+    uint64_t r = 1, x = b % p;
+    while (e) {
+        if (e & 1) r = r * x % p;
+        x = x * x % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+uint32_t shoup_q(uint32_t w, uint32_t p) { return (uint32_t)(((uint64_t)w << 32) / p); }
+uint32_t to_mont(uint32_t x, uint32_t p) { return (uint32_t)(((uint64_t)x << 32) % p); }
+
+PrimeK make_prime(int idx) {
+    PrimeK k{};
+    const uint32_t p = PRIME[idx];
+    uint32_t x = 1;
+    for (int i = 0; i < 5; ++i) x *= 2u - p * x;   // p^-1 mod 2^32
+    k.p = p;
+    k.ninv = 0u - x;
+    const uint32_t w = powmod(GEN[idx], (p - 1) / 32, p);
+    const uint32_t iw = powmod(w, p - 2, p);
+    uint32_t a = 1, b = 1;
+    for (int i = 0; i < 16; ++i) {
+        k.w32[i] = a; k.w32q[i] = shoup_q(a, p);
+        k.iw32[i] = b; k.iw32q[i] = shoup_q(b, p);
+        a = (uint64_t)a * w % p;
+        b = (uint64_t)b * iw % p;
+    }
+    return k;
+}
+
+// ---- pool --------------------------------------------------------------------
+
+size_t Pool::round(size_t b) {
+    if (b < 4096) return 4096;
+    // power-of-two classes split in eighths: waste below 12.5%
+    size_t top = 1;
+    while ((top << 1) <= b) top <<= 1;
+    const size_t step = std::max<size_t>(top >> 3, 4096);
+    return (b + step - 1) / step * step;
+}
+
+void *Pool::get(size_t bytes) {
+    const size_t r = round(bytes);
+    auto it = free_.find(r);
+    void *p = nullptr;
+    if (it != free_.end()) {
+        p = it->second;
+        free_.erase(it);
+    } else {
+        if (reserved_ + r > budget_ && !free_.empty()) {
+            // cached blocks may still be named by queued work: drain first
+            if (stream_) hipStreamSynchronize(stream_);
+            for (auto &f : free_) {
+                hipFree(f.second);
+                reserved_ -= f.first;
+            }
+            free_.clear();
+        }
+        if (reserved_ + r > budget_) return nullptr;
+        if (hipMalloc(&p, r) != hipSuccess) return nullptr;
+        reserved_ += r;
+    }
+    live_[p] = r;
+    in_use_ += r;
+    return p;
+}
+
+void Pool::put(void *p) {
+    if (!p) return;
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    in_use_ -= it->second;
+    free_.insert({it->second, p});
+    live_.erase(it);
+}
+
+void Pool::release_all() {
+    for (auto &f : free_) hipFree(f.second);
+    for (auto &l : live_) hipFree(l.first);
+    free_.clear();
+    live_.clear();
+    reserved_ = in_use_ = 0;
+}
+
+// ---- context --------------------------------------------------------------------
+
+static Context g_ctx;
+Context &ctx() { return g_ctx; }
+
+int Context::init(int dev, size_t budget) {
+    std::lock_guard<std::recursive_mutex> lk(mu);
+    if (device == dev && stream) return PG_OK;
+    if (stream) shutdown();
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (dev < 0 || dev >= n) return set_error(PG_EINVAL, "pg_init: no such device");
+    HIP_TRY(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(PG_EDEVICE, std::string("pg_init: built for gfx950, device is ") + prop.gcnArchName);
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    pool.set_budget(budget ? budget : fr / 10 * 9);
+    pool.set_stream(stream);
+    for (int i = 0; i < 3; ++i) pk[i] = make_prime(i);
+    crt_inv = powmod(PRIME[0], PRIME[1] - 2, PRIME[1]);
+    crt_invq = shoup_q(crt_inv, PRIME[1]);
+    const size_t lds_max = 160 * 1024;
+    HIP_TRY(hipFuncSetAttribute((const void *)k_col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_row, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    HIP_TRY(hipFuncSetAttribute((const void *)k_col_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    device = dev;
+    return PG_OK;
+}
+
+void Context::shutdown() {
+    std::lock_guard<std::recursive_mutex> lk(mu);
+    if (!stream) return;
+    hipStreamSynchronize(stream);
+    collect_timing();
+    for (auto e : event_pool) hipEventDestroy(e);
+    event_pool.clear();
+    for (void *p : tables_) hipFree(p);
+    tables_.clear();
+    tw_.clear();
+    gtw_.clear();
+    pool.release_all();
+    hipStreamDestroy(stream);
+    stream = nullptr;
+    device = -1;
+}
+
+void *Context::table_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    tables_.push_back(p);
+    return p;
+}
+
+// Table of pass (logB, lr): entry [k1][j2] = omega_B^(+-j2*k1) with Shoup quotient.
+const TwTables &Context::tw(int prime, int logB, int lr) {
+    const long key = ((long)prime << 16) | (logB << 8) | lr;
+    auto it = tw_.find(key);
+    if (it != tw_.end()) return it->second;
+    TwTables t;
+    const int R = 1 << lr, M = 1 << (logB - lr);
+    if (M > 1) {
+        const uint32_t p = PRIME[prime];
+        const uint32_t w = powmod(GEN[prime], (p - 1) >> logB, p);
+        const uint32_t iw = powmod(w, p - 2, p);
+        std::vector<uint2> f((size_t)R * M), v((size_t)R * M);
+        for (int k1 = 0; k1 < R; ++k1) {
+            const uint32_t wk = powmod(w, k1, p), iwk = powmod(iw, k1, p);
+            uint32_t a = 1, b = 1;
+            for (int j2 = 0; j2 < M; ++j2) {
+                f[(size_t)k1 * M + j2] = make_uint2(a, shoup_q(a, p));
+                v[(size_t)k1 * M + j2] = make_uint2(b, shoup_q(b, p));
+                a = (uint64_t)a * wk % p;
+                b = (uint64_t)b * iwk % p;
+            }
+        }
+        uint2 *df = (uint2 *)table_alloc(f.size() * sizeof(uint2));
+        uint2 *dv = (uint2 *)table_alloc(v.size() * sizeof(uint2));
+        if (df && dv) {
+            hipMemcpy(df, f.data(), f.size() * sizeof(uint2), hipMemcpyHostToDevice);
+            hipMemcpy(dv, v.data(), v.size() * sizeof(uint2), hipMemcpyHostToDevice);
+        }
+        t.fwd = df;
+        t.inv = dv;
+    }
+    return tw_[key] = t;
+}
+
+// Four-step twiddles omega_n^e = lo[e mod 2^s] * hi[e >> s] in Montgomery form;
+// the inverse low table carries the scale n^-1 * 2^32 that undoes both the
+// unnormalized inverse passes and the 2^-32 of the pointwise Montgomery product.
+const GlobalTw &Context::gtw(int prime, int logn) {
+    const long key = ((long)prime << 16) | logn;
+    auto it = gtw_.find(key);
+    if (it != gtw_.end()) return it->second;
+    GlobalTw g;
+    const uint32_t p = PRIME[prime];
+    const int s = (logn + 1) / 2;
+    const size_t nlo = (size_t)1 << s, nhi = (size_t)1 << (logn - s);
+    const uint32_t w = powmod(GEN[prime], (p - 1) >> logn, p);
+    const uint32_t iw = powmod(w, p - 2, p);
+    const uint32_t ninv_n = powmod((uint32_t)(((uint64_t)1 << logn) % p), p - 2, p);
+    const uint32_t scale = (uint32_t)((uint64_t)ninv_n * to_mont(1, p) % p);
+    std::vector<uint32_t> lo(nlo), hi(nhi), ilo(nlo), ihi(nhi);
+    uint32_t a = 1, b = 1;
+    for (size_t e = 0; e < nlo; ++e) {
+        lo[e] = to_mont(a, p);
+        ilo[e] = to_mont((uint32_t)((uint64_t)b * scale % p), p);
+        a = (uint64_t)a * w % p;
+        b = (uint64_t)b * iw % p;
+    }
+    const uint32_t ws = powmod(w, nlo, p), iws = powmod(iw, nlo, p);
+    a = b = 1;
+    for (size_t e = 0; e < nhi; ++e) {
+        hi[e] = to_mont(a, p);
+        ihi[e] = to_mont(b, p);
+        a = (uint64_t)a * ws % p;
+        b = (uint64_t)b * iws % p;
+    }
+    uint32_t *d = (uint32_t *)table_alloc((2 * nlo + 2 * nhi) * sizeof(uint32_t));
+    if (d) {
+        hipMemcpy(d, lo.data(), nlo * 4, hipMemcpyHostToDevice);
+        hipMemcpy(d + nlo, ilo.data(), nlo * 4, hipMemcpyHostToDevice);
+        hipMemcpy(d + 2 * nlo, hi.data(), nhi * 4, hipMemcpyHostToDevice);
+        hipMemcpy(d + 2 * nlo + nhi, ihi.data(), nhi * 4, hipMemcpyHostToDevice);
+        g.glo = d;
+        g.ilo = d + nlo;
+        g.ghi = d + 2 * nlo;
+        g.ihi = d + 2 * nlo + nhi;
+    }
+    g.lobits = s;
+    return gtw_[key] = g;
+}
+
+static SubPlan split(int logN) {
+    SubPlan sp{};
+    sp.logN = logN;
+    sp.npass = logN ? (logN + 4) / 5 : 0;
+    for (int i = 0; i < sp.npass; ++i) sp.lr[i] = logN / sp.npass + (i < logN % sp.npass ? 1 : 0);
+    return sp;
+}
+
+// n = N1 * N2: rows of up to 2^15 points in LDS; columns short enough that a
+// tile of T >= 8 of them fits a 64 KiB LDS tile.
+NttPlan Context::ntt_plan(int logn) {
+    NttPlan p;
+    p.logn = logn;
+    if (logn <= 15) {
+        p.logN2 = logn;
+    } else {
+        p.logN2 = std::min(15, std::max((logn + 1) / 2, logn - 10));
+    }
+    const int l1 = logn - p.logN2;
+    p.col = split(l1);
+    p.row = split(p.logN2);
+    p.logT = std::min(6, 14 - l1);
+    if (p.logT > p.logN2) p.logT = p.logN2;
+    return p;
+}
+
+// ---- timing -------------------------------------------------------------------
+
+void Context::stage_begin(hipEvent_t &a) {
+    if (!timing) return;
+    if (event_pool.empty()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        event_pool.push_back(e);
+    }
+    a = event_pool.back();
+    event_pool.pop_back();
+    hipEventRecord(a, stream);
+}
+
+void Context::stage_end(const char *name, hipEvent_t a, double bytes) {
+    if (!timing) return;
+    if (event_pool.empty()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        event_pool.push_back(e);
+    }
+    hipEvent_t b = event_pool.back();
+    event_pool.pop_back();
+    hipEventRecord(b, stream);
+    pending.push_back({name, a, b, bytes});
+    if (pending.size() > 4096) collect_timing();
+}
+
+void Context::collect_timing() {
+    for (auto &p : pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto &s = stats[p.name];
+        s.launches++;
+        s.ms += ms;
+        s.bytes += p.bytes;
+        event_pool.push_back(p.a);
+        event_pool.push_back(p.b);
+    }
+    pending.clear();
+}
+
+// ---- NTT product driver ----------------------------------------------------
+
+static int ndig16(uint64_t top) {
+    int d = 0;
+    while (top) { ++d; top >>= 16; }
+    return d;
+}
+
+struct Scratch {
+    std::vector<void *> bufs;
+    ~Scratch() {
+        for (void *p : bufs) ctx().pool.put(p);
+    }
+    template <class T>
+    T *get(size_t n) {
+        void *p = ctx().pool.get(n * sizeof(T));
+        if (p) bufs.push_back(p);
+        return (T *)p;
+    }
+};
+
+int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *out, size_t out_len,
+                 Scratch &s) {
+    Context &C = ctx();
+    CarryLaunch cl{};
+    cl.slo = slo;
+    cl.shi = shi;
+    cl.m = m;
+    cl.out = out;
+    cl.out_len = out_len;
+    cl.nchunks = (std::max(m, out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
+    cl.agg = s.get<uint8_t>(cl.nchunks);
+    cl.cin = s.get<uint8_t>(cl.nchunks);
+    if (!cl.agg || !cl.cin) return set_error(PG_ENOMEM, "carry scratch");
+    hipEvent_t e0 = nullptr;
+    C.stage_begin(e0);
+    hipLaunchKernelGGL(k_carry_reduce, dim3(cl.nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
+    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, C.stream, cl);
+    hipLaunchKernelGGL(k_carry_apply, dim3(cl.nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
+    C.stage_end("carry", e0, (double)m * 32.0 + (double)out_len * 8.0);
+    HIP_TRY(hipGetLastError());
+    return PG_OK;
+}
+
+int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,
+                   size_t bn, uint64_t btop) {
+    Context &C = ctx();
+    if (!C.stream) return set_error(PG_ENOINIT, "pg_init not called");
+    if (!an || !bn || !atop || !btop) return set_error(PG_EINVAL, "operands must be trimmed and nonzero");
+    const size_t da = 4 * (an - 1) + ndig16(atop), db = 4 * (bn - 1) + ndig16(btop);
+    if (std::max(an, bn) > ((size_t)1 << 24)) return set_error(PG_ERANGE, "NTT limit 2^24 limbs");
+    int logn = 0;
+    while (((size_t)1 << logn) < da + db) ++logn;
+    if (logn > 26) return set_error(PG_ERANGE, "NTT limit 2^26 points");
+    if (logn < 3) logn = 3;
+    const NttPlan P = Context::ntt_plan(logn);
+    const size_t n = (size_t)1 << logn;
+    const int NP = 2;
+    Scratch s;
+    NttLaunch L{};
+    L.a = ap; L.b = bp; L.an = an; L.bn = bn;
+    L.logn = logn; L.logN2 = P.logN2; L.logT = P.logT; L.nprimes = NP;
+    L.col = P.col; L.row = P.row;
+    L.crt_inv = C.crt_inv; L.crt_invq = C.crt_invq;
+    for (int k = 0; k < NP; ++k) {
+        L.pk[k] = C.pk[k];
+        int logB = P.col.logN;
+        for (int i = 0; i < P.col.npass; ++i) {
+            const TwTables &t = C.tw(k, logB, P.col.lr[i]);
+            L.col_tw[k][i] = t.fwd; L.col_itw[k][i] = t.inv;
+            logB -= P.col.lr[i];
+        }
+        logB = P.row.logN;
+        for (int i = 0; i < P.row.npass; ++i) {
+            const TwTables &t = C.tw(k, logB, P.row.lr[i]);
+            L.row_tw[k][i] = t.fwd; L.row_itw[k][i] = t.inv;
+            logB -= P.row.lr[i];
+        }
+        const GlobalTw &g = C.gtw(k, logn);
+        if (!g.glo) return set_error(PG_ENOMEM, "twiddle tables");
+        L.glo[k] = g.glo; L.ghi[k] = g.ghi; L.ilo[k] = g.ilo; L.ihi[k] = g.ihi; L.lobits = g.lobits;
+    }
+    if (P.col.npass) L.spec = s.get<uint32_t>(2 * NP * n);
+    L.prod = s.get<uint32_t>(NP * n);
+    const size_t m = n / 4;
+    L.slo = s.get<uint64_t>(m);
+    L.shi = s.get<uint64_t>(m);
+    if ((P.col.npass && !L.spec) || !L.prod || !L.slo || !L.shi) return set_error(PG_ENOMEM, "NTT scratch");
+
+    const int N1 = 1 << P.col.logN, N2 = 1 << P.row.logN, T = 1 << P.logT;
+    const int ntiles = N2 / T;
+    const size_t col_lds = (size_t)N1 * T * 4, row_lds = (size_t)N2 * 4;
+    hipEvent_t e0 = nullptr;
+    if (P.col.npass) {
+        C.stage_begin(e0);
+        hipLaunchKernelGGL(k_col_fwd, dim3(ntiles, 2, NP), dim3(NTT_THREADS), col_lds, C.stream, L);
+        C.stage_end("ntt_col_fwd", e0, (double)(an + bn) * 8.0 * NP + 2.0 * NP * n * 4.0);
+    }
+    C.stage_begin(e0);
+    hipLaunchKernelGGL(k_row, dim3(N1, NP), dim3(NTT_THREADS), row_lds, C.stream, L);
+    C.stage_end("ntt_row", e0, (P.col.npass ? 2.0 * NP * n * 4.0 : (double)(an + bn) * 8.0 * NP) + NP * n * 4.0);
+    C.stage_begin(e0);
+    hipLaunchKernelGGL(k_col_inv, dim3(ntiles), dim3(NTT_THREADS), col_lds, C.stream, L);
+    C.stage_end("ntt_col_inv", e0, NP * n * 4.0 + m * 16.0);
+    HIP_TRY(hipGetLastError());
+    return launch_carry(L.slo, L.shi, m, rp, an + bn, s);
+}
+
+}  // namespace pg

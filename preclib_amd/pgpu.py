@@ -1,0 +1,175 @@
+"""ctypes binding of libpgpu.so (the C ABI in include/pgpu.h).
+
+The product path runs only through the HIP library: if libpgpu.so is missing
+or no gfx950 device can be opened, every call raises PgError.  There is no
+CPU fallback here.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpgpu.so")
+
+PG_OK, PG_EINVAL, PG_ENOMEM, PG_EDEVICE, PG_ERANGE, PG_ENOINIT = 0, -1, -2, -3, -4, -5
+PATH_AUTO, PATH_NTT, PATH_FFT = 0, 1, 2
+P64 = ctypes.POINTER(ctypes.c_uint64)
+SZ = ctypes.c_size_t
+
+# Every symbol include/pgpu.h declares.
+EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", "pg_buf_free",
+           "pg_buf_capacity", "pg_buf_upload", "pg_buf_download", "pg_buf_ptr", "pg_mul_dev",
+           "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read"]
+
+
+class PgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pgpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+class StageTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("launches", ctypes.c_uint64),
+                ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libpgpu.so (no device call)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PgError(PG_ENOINIT, "libpgpu.so not built (python -m preclib_amd.build)")
+    L = ctypes.CDLL(path)
+    L.pg_init.argtypes = [ctypes.c_int, SZ]
+    L.pg_last_error.restype = ctypes.c_char_p
+    L.pg_mul.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, P64, SZ, ctypes.c_int,
+                         ctypes.POINTER(ctypes.c_double)]
+    L.pg_buf_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), SZ]
+    L.pg_buf_free.argtypes = [ctypes.c_void_p]
+    L.pg_buf_capacity.argtypes = [ctypes.c_void_p]
+    L.pg_buf_capacity.restype = SZ
+    L.pg_buf_ptr.argtypes = [ctypes.c_void_p]
+    L.pg_buf_ptr.restype = ctypes.c_void_p
+    L.pg_buf_upload.argtypes = [ctypes.c_void_p, P64, SZ]
+    L.pg_buf_download.argtypes = [ctypes.c_void_p, P64, SZ]
+    L.pg_mul_dev.argtypes = [ctypes.c_void_p, ctypes.POINTER(SZ), ctypes.c_void_p, SZ, ctypes.c_void_p,
+                             SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.pg_timing_enable.argtypes = [ctypes.c_int]
+    L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
+    _lib = L
+    return L
+
+
+def _check(code):
+    if code != PG_OK:
+        msg = _lib.pg_last_error()
+        raise PgError(code, msg.decode() if msg else "")
+    return code
+
+
+_inited = False
+
+
+def init(device=0, budget=0):
+    global _inited
+    L = load()
+    _check(L.pg_init(device, budget))
+    _inited = True
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P64)
+
+
+def _trimmed(x):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    n = len(x)
+    while n and x[n - 1] == 0:
+        n -= 1
+    return x[:n]
+
+
+def mul(a, b, path=PATH_AUTO, want_round=False):
+    """Product of two limb arrays on the GPU (host buffers in and out)."""
+    if not _inited:
+        init()
+    a, b = _trimmed(a), _trimmed(b)
+    r = np.zeros(len(a) + len(b), np.uint64)
+    rl = SZ(0)
+    err = ctypes.c_double(0.0)
+    _check(_lib.pg_mul(_ptr(r), len(r), ctypes.byref(rl), _ptr(a), len(a), _ptr(b), len(b), path,
+                       ctypes.byref(err)))
+    r = r[: rl.value]
+    return (r, err.value) if want_round else r
+
+
+class DevBuf:
+    """A limb array resident in HBM (pg_buf handle)."""
+
+    def __init__(self, limbs):
+        if not _inited:
+            init()
+        h = ctypes.c_void_p()
+        _check(_lib.pg_buf_alloc(ctypes.byref(h), limbs))
+        self.h = h
+        self.cap = limbs
+
+    @classmethod
+    def of(cls, x):
+        x = np.ascontiguousarray(x, dtype=np.uint64)
+        d = cls(max(len(x), 1))
+        d.upload(x)
+        return d
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=np.uint64)
+        _check(_lib.pg_buf_upload(self.h, _ptr(x), len(x)))
+
+    def download(self, n):
+        out = np.zeros(n, np.uint64)
+        _check(_lib.pg_buf_download(self.h, _ptr(out), n))
+        return out
+
+    def free(self):
+        if self.h:
+            _lib.pg_buf_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def mul_dev(r, a, an, b, bn, path=PATH_AUTO, want_len=True):
+    rl = SZ(0)
+    err = ctypes.c_double(0.0)
+    _check(_lib.pg_mul_dev(r.h, ctypes.byref(rl) if want_len else None, a.h, an, b.h, bn, path,
+                           ctypes.byref(err)))
+    return rl.value, err.value
+
+
+def sync():
+    _check(_lib.pg_sync())
+
+
+def timing(on=True):
+    load().pg_timing_enable(1 if on else 0)
+
+
+def timing_reset():
+    load().pg_timing_reset()
+
+
+def timing_read():
+    L = load()
+    n = L.pg_timing_read(None, 0)
+    arr = (StageTime * max(n, 1))()
+    n = L.pg_timing_read(arr, n)
+    return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms, arr[i].bytes) for i in range(n)}

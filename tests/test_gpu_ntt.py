@@ -1,0 +1,86 @@
+"""GPU parity of the NTT product path (pg_mul through the C ABI) against the
+oracle and the GMP digests; bit-exact, limb for limb."""
+import numpy as np
+import pytest
+
+from preclib_amd import workloads as W
+from tests.oracle_py import digest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pg():
+    from preclib_amd import pgpu
+    pgpu.init(0)
+    return pgpu
+
+
+def _wl(golden, an, bn, dist):
+    return next(e for e in golden["workload"] if e["an"] == an and e["bn"] == bn and e["dist"] == dist)
+
+
+@pytest.mark.parametrize("shape", [(193, 193), (300, 255), (1000, 1000), (2049, 2039), (4097, 4097),
+                                   (6145, 6145), (6145, 12000), (9000, 5000), (16384, 12289)])
+@pytest.mark.parametrize("dist", ["uniform", "ones", "sparse"])
+def test_ntt_vs_oracle(pg, oracle, shape, dist):
+    an, bn = shape
+    a, b = W.pair(an, bn, dist, k=an)
+    print("case", an, bn, dist, flush=True)
+    got = pg.mul(a, b, path=pg.PATH_NTT)
+    want = oracle.mul_ntt(a, b)
+    assert len(got) == len(want)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "ones", "sparse"])
+def test_ntt_16384_golden(pg, golden, dist):
+    a, b = W.pair(16384, 16384, dist)
+    got = pg.mul(a, b)
+    e = _wl(golden, 16384, 16384, dist)
+    assert len(got) == e["len"] and digest(got) == e["sha256"]
+
+
+@pytest.mark.parametrize("logn", [20, 22])
+@pytest.mark.parametrize("dist", ["uniform", "ones", "sparse"])
+def test_ntt_large_golden(pg, golden, logn, dist):
+    n = 1 << logn
+    a, b = W.pair(n, n, dist)
+    print("case", n, dist, flush=True)
+    got = pg.mul(a, b)
+    e = _wl(golden, n, n, dist)
+    assert len(got) == e["len"] and digest(got) == e["sha256"]
+    if dist == "ones":   # (2^64N - 1)^2 closed form
+        assert got[0] == 1 and not got[1:n].any()
+        assert got[n] == 0xFFFFFFFFFFFFFFFE and (got[n + 1:] == 0xFFFFFFFFFFFFFFFF).all()
+
+
+def test_ntt_torture_patterns(pg, golden):
+    from tests.golden.gen_golden import TORTURE_PATTERNS, torture_operands
+    pats = {p[0]: p for p in TORTURE_PATTERNS}
+    for e in golden["torture"]:
+        a, b = torture_operands(e["an"], e["bn"], pats[e["pattern"]])
+        if max(len(a), len(b)) <= 192:
+            continue
+        got = pg.mul(a, b, path=pg.PATH_NTT)
+        assert digest(got) == e["sha256"], e
+
+
+def test_device_resident_alias(pg, oracle):
+    a, b = W.pair(20000, 15000, "uniform", k=7)
+    want = oracle.mul_ntt(a, b)
+    da = pg.DevBuf(len(a) + len(b))
+    da.upload(a)
+    db = pg.DevBuf.of(b)
+    n, _ = pg.mul_dev(da, da, len(a), db, len(b), path=pg.PATH_NTT)   # r aliases a
+    assert n == len(want)
+    assert np.array_equal(da.download(n), want)
+    da.free()
+    db.free()
+
+
+def test_repeat_is_deterministic(pg):
+    a, b = W.pair(50000, 50000, "uniform", k=3)
+    r1 = pg.mul(a, b)
+    r2 = pg.mul(a, b)
+    assert np.array_equal(r1, r2)
