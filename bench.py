@@ -31,7 +31,8 @@ NPRIMES = 2
 
 
 def model_bytes(an, bn, n):
-    """Per-kernel share of the byte model ('Byte model')."""
+    """Per-kernel share of the reference byte model B = 16(Na+Nb) + 48 P n with
+    the reference's P = 2 and its transform length n (base-2^16 digits)."""
     P = NPRIMES
     return {
         "ntt_col_fwd": 8.0 * (an + bn) + 16.0 * P * n,

@@ -33,7 +33,11 @@ enum {
  * hi > 6144, FFT otherwise.  PG_PATH_NTT / PG_PATH_FFT force the transform
  * (mul_ntt / mul_fft semantics, including mul_ntt's fall back to the FFT
  * above 2^24 limbs or 2^26 points, mul_ntt.cpp:497-506). */
-enum { PG_PATH_AUTO = 0, PG_PATH_NTT = 1, PG_PATH_FFT = 2 };
+enum { PG_PATH_AUTO = 0, PG_PATH_NTT = 1, PG_PATH_FFT = 2, PG_PATH_NTT_REF = 3 };
+/* PG_PATH_NTT runs the transform in the library's fastest digit mode (base
+ * 2^32 digits over all three of preclib's primes); PG_PATH_NTT_REF runs
+ * preclib's own decomposition (base-2^16 digits, primes 469762049 and
+ * 1811939329, mul_ntt.cpp:331-342, 509-513).  Both return the same product. */
 
 /* Opens device `device` and sets the device-memory budget in bytes (0 = 90%
  * of free HBM).  Replaces nothing in preclib (it has no device); must precede

@@ -186,8 +186,8 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
         if (hi > 2 * lo) return set_error(PG_ERANGE, "blocked path not available yet");
         path = hi > 6144 ? PG_PATH_NTT : PG_PATH_FFT;
     }
-    if (path == PG_PATH_NTT) {
-        int st = ntt_mul_device(rp, ap, an, atop, bp, bn, btop);
+    if (path == PG_PATH_NTT || path == PG_PATH_NTT_REF) {
+        int st = ntt_mul_device(rp, ap, an, atop, bp, bn, btop, path == PG_PATH_NTT_REF ? MODE_REF : MODE_WIDE);
         if (st != PG_ERANGE) return st;
         path = PG_PATH_FFT;   // beyond the NTT caps the reference multiplies with mul_fft
     }

@@ -7,16 +7,58 @@
 
 namespace pg {
 
-constexpr int NTT_THREADS = 1024;           // workgroup size of the NTT kernels
+constexpr int NTT_THREADS = 1024;           // workgroup size of the column kernels
+constexpr int COLINV_THREADS = 512;         // workgroup size of the inverse column kernel
 constexpr int ROW_MAX_ELEMS = 1 << 15;      // longest in-LDS row transform (N2)
 constexpr int COL_MAX_ELEMS = 1 << 14;      // largest column tile (N1 * T)
 constexpr int MAX_PASS = 4;
 
-// A power-of-two transform done in LDS as radix-2^lr[i] passes.
-struct SubPlan {
-    int logN;
-    int npass;
-    int lr[MAX_PASS];
+// LDS words for n elements with one pad word per 32 (see lp() in pg_ntt.hip).
+constexpr size_t lds_words(size_t n) { return n + (n >> 5); }
+
+// ---- transform plan, shared by host and kernels ---------------------------
+// n = 2^logn = N1 * N2: rows of N2 = 2^plan_l2 points are transformed in LDS
+// by the row kernel; columns of N1 points in tiles of T = 2^plan_lt columns.
+constexpr int NTT_MIN_LOG = 8, NTT_MAX_LOG = 26;
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+#ifndef PG_ROW_LOG_MAX
+#define PG_ROW_LOG_MAX 15
+#endif
+constexpr int ROW_LOG_MAX = PG_ROW_LOG_MAX;
+constexpr int plan_l2(int logn) {
+    return logn <= ROW_LOG_MAX ? logn : cmin(ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 10));
+}
+constexpr int plan_l1(int logn) { return logn - plan_l2(logn); }
+constexpr int plan_lt(int logn) { return cmin(cmin(6, 14 - plan_l1(logn)), plan_l2(logn)); }
+// An in-LDS transform of 2^l points as ceil(l/5) radix passes of near-equal
+// size (largest first).
+// Row workgroup: 512 threads, 1024 for 2^15-point rows (32 kept values each).
+constexpr int row_threads(int logn) { return plan_l2(logn) >= 15 ? 1024 : 512; }
+constexpr int split_np(int l) { return l ? (l + 4) / 5 : 0; }
+constexpr int split_lr(int l, int i) {
+    return split_np(l) == 0 ? 0 : (i < split_np(l) ? l / split_np(l) + (i < l % split_np(l) ? 1 : 0) : 0);
+}
+
+// Digit modes of the NTT product.
+//  MODE_REF  : preclib's own decomposition (mul_ntt.cpp:331-342, 509-513):
+//              base-2^16 digits, primes p1, p2, two-prime Garner CRT.
+//  MODE_WIDE : base-2^32 digits over all three primes of the same set
+//              (mul_ntt.cpp:29-31).  A coefficient is below
+//              2^25 * (2^32 - 1)^2 < 2^89 < p1 p2 p3 (~2^90.47) for every size
+//              the NTT path accepts (<= 2^24 limbs), so the three-prime CRT is
+//              exact; the transforms are half as long with 3/2 as many primes.
+enum { MODE_REF = 0, MODE_WIDE = 1 };
+constexpr int mode_primes(int mode) { return mode == MODE_WIDE ? 3 : 2; }
+constexpr int mode_digit_bits(int mode) { return mode == MODE_WIDE ? 32 : 16; }
+
+struct CrtK {
+    // two-prime Garner (mul_ntt.cpp:405-412): t = (r2 - r1) * p1^-1 mod p2
+    uint32_t inv12, inv12q;
+    // three-prime Garner: x12 = r1 + p1 t2;  t3 = (r3 - x12) * (p1 p2)^-1 mod p3
+    uint32_t p1_mod3, p1_mod3q;       // p1 mod p3 (Shoup)
+    uint32_t inv123, inv123q;         // (p1 p2)^-1 mod p3 (Shoup)
+    uint32_t red[3];                  // floor(2^32 / p_k): Shoup constants of x * 1 mod p_k
 };
 
 struct NttLaunch {
@@ -24,19 +66,24 @@ struct NttLaunch {
     size_t an, bn;              // limb counts
     uint32_t *spec;             // forward spectra [2 operands][nprimes][n]
     uint32_t *prod;             // product spectra / residues [nprimes][n]
-    uint64_t *slo, *shi;        // 128-bit limb sums [n/4]
-    int logn, logN2, logT, lobits, nprimes;
-    SubPlan col, row;
-    PrimeK pk[2];
-    const uint2 *col_tw[2][MAX_PASS], *col_itw[2][MAX_PASS];
-    const uint2 *row_tw[2][MAX_PASS], *row_itw[2][MAX_PASS];
-    const uint32_t *glo[2], *ghi[2], *ilo[2], *ihi[2];
-    uint32_t crt_inv, crt_invq;   // p1^-1 mod p2 and its Shoup quotient
+    uint64_t *slo, *shi;        // 128-bit limb sums [number of product limbs]
+    int logn, lobits, nprimes, mode;
+    PrimeK pk[3];
+    CrtK crt;
+    const uint2 *col_tw[3][MAX_PASS], *col_itw[3][MAX_PASS];
+    const uint2 *row_tw[3][MAX_PASS], *row_itw[3][MAX_PASS];
+    const uint32_t *glo[3], *ghi[3], *ilo[3], *ihi[3];
 };
 
-__global__ void k_col_fwd(const NttLaunch L);
-__global__ void k_row(const NttLaunch L);
-__global__ void k_col_inv(const NttLaunch L);
+// Enqueues the three transform kernels of the plan for L.logn on `stream`;
+// per-stage timing hooks are called around each launch.
+struct StageHook {
+    void (*begin)(void *ctx, const char *name);
+    void (*end)(void *ctx, const char *name, double bytes);
+    void *ctx;
+};
+hipError_t launch_ntt(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
+hipError_t ntt_set_lds_limits();
 
 // ---- carry propagation (pg_carry.hip) ----------------------------------------
 constexpr int CARRY_THREADS = 256;

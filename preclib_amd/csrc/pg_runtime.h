@@ -55,11 +55,6 @@ struct GlobalTw {
     int lobits = 0;
 };
 
-struct NttPlan {
-    int logn = 0, logN2 = 0, logT = 0;
-    SubPlan col{}, row{};
-};
-
 struct StageStat {
     uint64_t launches = 0;
     double ms = 0.0, bytes = 0.0;
@@ -72,14 +67,13 @@ public:
     Pool pool;
     std::recursive_mutex mu;
     PrimeK pk[3];
-    uint32_t crt_inv = 0, crt_invq = 0;
+    CrtK crt{};
 
     int init(int dev, size_t budget);
     void shutdown();
 
     const TwTables &tw(int prime, int logB, int lr);
     const GlobalTw &gtw(int prime, int logn);
-    static NttPlan ntt_plan(int logn);
 
     // timing
     bool timing = false;
@@ -94,6 +88,8 @@ public:
     void stage_begin(hipEvent_t &a);
     void stage_end(const char *name, hipEvent_t a, double bytes);
     void collect_timing();
+    hipEvent_t hook_event = nullptr;
+    StageHook hook();
 
 private:
     std::map<long, TwTables> tw_;
@@ -108,6 +104,6 @@ int set_error(int code, const std::string &msg);
 // Product entry points on device pointers (stream-ordered on ctx().stream).
 // rp must hold an + bn limbs; it may alias ap or bp.  Returns a status code.
 int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
-                   const uint64_t *bp, size_t bn, uint64_t btop);
+                   const uint64_t *bp, size_t bn, uint64_t btop, int mode);
 
 }  // namespace pg

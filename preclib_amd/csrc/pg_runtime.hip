@@ -130,12 +130,14 @@ int Context::init(int dev, size_t budget) {
     pool.set_budget(budget ? budget : fr / 10 * 9);
     pool.set_stream(stream);
     for (int i = 0; i < 3; ++i) pk[i] = make_prime(i);
-    crt_inv = powmod(PRIME[0], PRIME[1] - 2, PRIME[1]);
-    crt_invq = shoup_q(crt_inv, PRIME[1]);
-    const size_t lds_max = 160 * 1024;
-    HIP_TRY(hipFuncSetAttribute((const void *)k_col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    HIP_TRY(hipFuncSetAttribute((const void *)k_row, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-    HIP_TRY(hipFuncSetAttribute((const void *)k_col_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+    crt.inv12 = powmod(PRIME[0], PRIME[1] - 2, PRIME[1]);
+    crt.inv12q = shoup_q(crt.inv12, PRIME[1]);
+    crt.p1_mod3 = PRIME[0] % PRIME[2];
+    crt.p1_mod3q = shoup_q(crt.p1_mod3, PRIME[2]);
+    crt.inv123 = powmod((uint32_t)((uint64_t)PRIME[0] * PRIME[1] % PRIME[2]), PRIME[2] - 2, PRIME[2]);
+    crt.inv123q = shoup_q(crt.inv123, PRIME[2]);
+    for (int k = 0; k < 3; ++k) crt.red[k] = (uint32_t)((((uint64_t)1) << 32) / PRIME[k]);
+    HIP_TRY(ntt_set_lds_limits());
     device = dev;
     return PG_OK;
 }
@@ -244,32 +246,6 @@ const GlobalTw &Context::gtw(int prime, int logn) {
     return gtw_[key] = g;
 }
 
-static SubPlan split(int logN) {
-    SubPlan sp{};
-    sp.logN = logN;
-    sp.npass = logN ? (logN + 4) / 5 : 0;
-    for (int i = 0; i < sp.npass; ++i) sp.lr[i] = logN / sp.npass + (i < logN % sp.npass ? 1 : 0);
-    return sp;
-}
-
-// n = N1 * N2: rows of up to 2^15 points in LDS; columns short enough that a
-// tile of T >= 8 of them fits a 64 KiB LDS tile.
-NttPlan Context::ntt_plan(int logn) {
-    NttPlan p;
-    p.logn = logn;
-    if (logn <= 15) {
-        p.logN2 = logn;
-    } else {
-        p.logN2 = std::min(15, std::max((logn + 1) / 2, logn - 10));
-    }
-    const int l1 = logn - p.logN2;
-    p.col = split(l1);
-    p.row = split(p.logN2);
-    p.logT = std::min(6, 14 - l1);
-    if (p.logT > p.logN2) p.logT = p.logN2;
-    return p;
-}
-
 // ---- timing -------------------------------------------------------------------
 
 void Context::stage_begin(hipEvent_t &a) {
@@ -297,6 +273,16 @@ void Context::stage_end(const char *name, hipEvent_t a, double bytes) {
     pending.push_back({name, a, b, bytes});
     if (pending.size() > 4096) collect_timing();
 }
+
+static void hook_begin(void *c, const char *) {
+    Context *C = (Context *)c;
+    C->stage_begin(C->hook_event);
+}
+static void hook_end(void *c, const char *name, double bytes) {
+    Context *C = (Context *)c;
+    C->stage_end(name, C->hook_event, bytes);
+}
+StageHook Context::hook() { return StageHook{hook_begin, hook_end, this}; }
 
 void Context::collect_timing() {
     for (auto &p : pending) {
@@ -358,66 +344,58 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
 }
 
 int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,
-                   size_t bn, uint64_t btop) {
+                   size_t bn, uint64_t btop, int mode) {
     Context &C = ctx();
     if (!C.stream) return set_error(PG_ENOINIT, "pg_init not called");
     if (!an || !bn || !atop || !btop) return set_error(PG_EINVAL, "operands must be trimmed and nonzero");
-    const size_t da = 4 * (an - 1) + ndig16(atop), db = 4 * (bn - 1) + ndig16(btop);
+    // mul_ntt's caps (mul_ntt.cpp:497-506), evaluated on its own base-2^16 digits
+    const size_t d16a = 4 * (an - 1) + ndig16(atop), d16b = 4 * (bn - 1) + ndig16(btop);
     if (std::max(an, bn) > ((size_t)1 << 24)) return set_error(PG_ERANGE, "NTT limit 2^24 limbs");
+    size_t n16 = 1;
+    while (n16 < d16a + d16b) n16 <<= 1;
+    if (n16 > ((size_t)1 << 26)) return set_error(PG_ERANGE, "NTT limit 2^26 points");
+    size_t da = d16a, db = d16b;
+    if (mode == MODE_WIDE) {
+        da = 2 * (an - 1) + ((atop >> 32) ? 2 : 1);
+        db = 2 * (bn - 1) + ((btop >> 32) ? 2 : 1);
+    }
     int logn = 0;
     while (((size_t)1 << logn) < da + db) ++logn;
-    if (logn > 26) return set_error(PG_ERANGE, "NTT limit 2^26 points");
-    if (logn < 3) logn = 3;
-    const NttPlan P = Context::ntt_plan(logn);
+    if (logn < NTT_MIN_LOG) logn = NTT_MIN_LOG;
+    if (logn > NTT_MAX_LOG) return set_error(PG_ERANGE, "transform too long");
     const size_t n = (size_t)1 << logn;
-    const int NP = 2;
+    const int NP = mode_primes(mode), DPL = 64 / mode_digit_bits(mode);
     Scratch s;
     NttLaunch L{};
     L.a = ap; L.b = bp; L.an = an; L.bn = bn;
-    L.logn = logn; L.logN2 = P.logN2; L.logT = P.logT; L.nprimes = NP;
-    L.col = P.col; L.row = P.row;
-    L.crt_inv = C.crt_inv; L.crt_invq = C.crt_invq;
+    L.logn = logn; L.nprimes = NP; L.mode = mode;
+    L.crt = C.crt;
+    const int l1 = plan_l1(logn), l2 = plan_l2(logn);
     for (int k = 0; k < NP; ++k) {
         L.pk[k] = C.pk[k];
-        int logB = P.col.logN;
-        for (int i = 0; i < P.col.npass; ++i) {
-            const TwTables &t = C.tw(k, logB, P.col.lr[i]);
+        int logB = l1;
+        for (int i = 0; i < split_np(l1); ++i) {
+            const TwTables &t = C.tw(k, logB, split_lr(l1, i));
             L.col_tw[k][i] = t.fwd; L.col_itw[k][i] = t.inv;
-            logB -= P.col.lr[i];
+            logB -= split_lr(l1, i);
         }
-        logB = P.row.logN;
-        for (int i = 0; i < P.row.npass; ++i) {
-            const TwTables &t = C.tw(k, logB, P.row.lr[i]);
+        logB = l2;
+        for (int i = 0; i < split_np(l2); ++i) {
+            const TwTables &t = C.tw(k, logB, split_lr(l2, i));
             L.row_tw[k][i] = t.fwd; L.row_itw[k][i] = t.inv;
-            logB -= P.row.lr[i];
+            logB -= split_lr(l2, i);
         }
         const GlobalTw &g = C.gtw(k, logn);
         if (!g.glo) return set_error(PG_ENOMEM, "twiddle tables");
         L.glo[k] = g.glo; L.ghi[k] = g.ghi; L.ilo[k] = g.ilo; L.ihi[k] = g.ihi; L.lobits = g.lobits;
     }
-    if (P.col.npass) L.spec = s.get<uint32_t>(2 * NP * n);
+    if (l1) L.spec = s.get<uint32_t>(2 * NP * n);
     L.prod = s.get<uint32_t>(NP * n);
-    const size_t m = n / 4;
+    const size_t m = n / DPL;
     L.slo = s.get<uint64_t>(m);
     L.shi = s.get<uint64_t>(m);
-    if ((P.col.npass && !L.spec) || !L.prod || !L.slo || !L.shi) return set_error(PG_ENOMEM, "NTT scratch");
-
-    const int N1 = 1 << P.col.logN, N2 = 1 << P.row.logN, T = 1 << P.logT;
-    const int ntiles = N2 / T;
-    const size_t col_lds = (size_t)N1 * T * 4, row_lds = (size_t)N2 * 4;
-    hipEvent_t e0 = nullptr;
-    if (P.col.npass) {
-        C.stage_begin(e0);
-        hipLaunchKernelGGL(k_col_fwd, dim3(ntiles, 2, NP), dim3(NTT_THREADS), col_lds, C.stream, L);
-        C.stage_end("ntt_col_fwd", e0, (double)(an + bn) * 8.0 * NP + 2.0 * NP * n * 4.0);
-    }
-    C.stage_begin(e0);
-    hipLaunchKernelGGL(k_row, dim3(N1, NP), dim3(NTT_THREADS), row_lds, C.stream, L);
-    C.stage_end("ntt_row", e0, (P.col.npass ? 2.0 * NP * n * 4.0 : (double)(an + bn) * 8.0 * NP) + NP * n * 4.0);
-    C.stage_begin(e0);
-    hipLaunchKernelGGL(k_col_inv, dim3(ntiles), dim3(NTT_THREADS), col_lds, C.stream, L);
-    C.stage_end("ntt_col_inv", e0, NP * n * 4.0 + m * 16.0);
-    HIP_TRY(hipGetLastError());
+    if ((l1 && !L.spec) || !L.prod || !L.slo || !L.shi) return set_error(PG_ENOMEM, "NTT scratch");
+    HIP_TRY(launch_ntt(L, C.stream, C.hook()));
     return launch_carry(L.slo, L.shi, m, rp, an + bn, s);
 }
 

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpgpu.so")
 
 PG_OK, PG_EINVAL, PG_ENOMEM, PG_EDEVICE, PG_ERANGE, PG_ENOINIT = 0, -1, -2, -3, -4, -5
-PATH_AUTO, PATH_NTT, PATH_FFT = 0, 1, 2
+PATH_AUTO, PATH_NTT, PATH_FFT, PATH_NTT_REF = 0, 1, 2, 3
 P64 = ctypes.POINTER(ctypes.c_uint64)
 SZ = ctypes.c_size_t
 
@@ -37,11 +37,12 @@ class StageTime(ctypes.Structure):
 _lib = None
 
 
-def load(path=LIB_PATH):
-    """Loads libpgpu.so (no device call)."""
+def load(path=None):
+    """Loads libpgpu.so (no device call).  PGPU_LIB overrides the path."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("PGPU_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise PgError(PG_ENOINIT, "libpgpu.so not built (python -m preclib_amd.build)")
     L = ctypes.CDLL(path)

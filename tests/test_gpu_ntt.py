@@ -23,11 +23,12 @@ def _wl(golden, an, bn, dist):
 @pytest.mark.parametrize("shape", [(193, 193), (300, 255), (1000, 1000), (2049, 2039), (4097, 4097),
                                    (6145, 6145), (6145, 12000), (9000, 5000), (16384, 12289)])
 @pytest.mark.parametrize("dist", ["uniform", "ones", "sparse"])
-def test_ntt_vs_oracle(pg, oracle, shape, dist):
+@pytest.mark.parametrize("mode", ["wide", "ref"])
+def test_ntt_vs_oracle(pg, oracle, shape, dist, mode):
     an, bn = shape
     a, b = W.pair(an, bn, dist, k=an)
-    print("case", an, bn, dist, flush=True)
-    got = pg.mul(a, b, path=pg.PATH_NTT)
+    print("case", an, bn, dist, mode, flush=True)
+    got = pg.mul(a, b, path=pg.PATH_NTT if mode == "wide" else pg.PATH_NTT_REF)
     want = oracle.mul_ntt(a, b)
     assert len(got) == len(want)
     assert np.array_equal(got, want)
@@ -43,11 +44,12 @@ def test_ntt_16384_golden(pg, golden, dist):
 
 @pytest.mark.parametrize("logn", [20, 22])
 @pytest.mark.parametrize("dist", ["uniform", "ones", "sparse"])
-def test_ntt_large_golden(pg, golden, logn, dist):
+@pytest.mark.parametrize("mode", ["wide", "ref"])
+def test_ntt_large_golden(pg, golden, logn, dist, mode):
     n = 1 << logn
     a, b = W.pair(n, n, dist)
-    print("case", n, dist, flush=True)
-    got = pg.mul(a, b)
+    print("case", n, dist, mode, flush=True)
+    got = pg.mul(a, b, path=pg.PATH_NTT if mode == "wide" else pg.PATH_NTT_REF)
     e = _wl(golden, n, n, dist)
     assert len(got) == e["len"] and digest(got) == e["sha256"]
     if dist == "ones":   # (2^64N - 1)^2 closed form
@@ -62,8 +64,9 @@ def test_ntt_torture_patterns(pg, golden):
         a, b = torture_operands(e["an"], e["bn"], pats[e["pattern"]])
         if max(len(a), len(b)) <= 192:
             continue
-        got = pg.mul(a, b, path=pg.PATH_NTT)
-        assert digest(got) == e["sha256"], e
+        for path in (pg.PATH_NTT, pg.PATH_NTT_REF):
+            got = pg.mul(a, b, path=path)
+            assert digest(got) == e["sha256"], (e, path)
 
 
 def test_device_resident_alias(pg, oracle):
