@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--logn", type=int, default=22, help="operand size 2^logn limbs")
     ap.add_argument("--dist", default="uniform")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the product path "
@@ -110,6 +112,13 @@ def main():
         sys.exit(2)
 
     from preclib_amd import pgpu, workloads
+    if args.copy_probe:
+        import torch
+        x = torch.ones(1 << 26, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     pgpu.init(0)
     N = 1 << args.logn
     a, b = workloads.pair(N, N, args.dist)
@@ -152,6 +161,11 @@ def main():
     avg_s = tot_ms / launches * 1e-3
     achieved = mb[dom] / avg_s
     prod_bytes = 16.0 * 2 * N + 48.0 * NPRIMES * n
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_2p22.json")
+    if args.logn == 22 and os.path.exists(tpath):
+        st = json.load(open(tpath))["stages"].get(dom)
+        traffic = st["traffic_bytes"] if st else None
     line = {
         "metric": "product limbs/s at 2^22 limbs (plain GPU NTT product, device-resident)",
         "value": value,
@@ -169,7 +183,7 @@ def main():
                    (N, N, n.bit_length() - 1, NPRIMES), "an": N, "bn": N, "dist": args.dist,
                    "bit_exact_vs_golden": check},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3},
         "product_roofline": {"bytes": prod_bytes, "achieved_GBps": prod_bytes / (ms * 1e-3) / 1e9,
                              "frac": prod_bytes / (ms * 1e-3) / HBM_PEAK},

@@ -2,25 +2,31 @@
 //
 // Replaces the serial carry loop of preclib's ntt_from_residues2
 // (src/mul_ntt.cpp:446-456) and fft_from_digits (src/mul_fft.cpp:311-338).
-// Input: limb sums S_j = lo_j + 2^64 hi_j with hi_j < 2^50, value
+// Input: limb sums S_j = lo_j + 2^64 hi_j with hi_j < 2^60, value
 // V = sum_j S_j 2^(64 j).  With t_j = lo_j + hi_{j-1}:
 //   g_j = carry out of that 64-bit add,  p_j = (t_j mod 2^64 == 2^64 - 1),
 //   x_{j+1} = g_j | (p_j & x_j),        limb_j = t_j + x_j  (mod 2^64).
-// g_j = 1 forces t_j < 2^50, so g and p never hold together and x is one bit.
-// (g, p) pairs compose associatively; a wave resolves 64 of them at once as
-// the carries of the 64-bit sum X + Y + cin with X = ballot(g|p), Y = ballot(g)
-// (carry into lane i = bit i of (X + Y + cin) ^ X ^ Y).  Three launches:
-// per-chunk reduce, one-workgroup scan of chunk aggregates, apply.
+// g_j = 1 forces t_j < 2^60, so g and p never hold together and x is one bit.
+//
+// A wave resolves 64 consecutive limbs at once: with X = ballot(g | p) and
+// Y = ballot(g), the carry into lane i is bit i of (X + Y + cin) ^ X ^ Y and
+// the carry out of the group is the overflow of that 64-bit sum.  Each wave
+// walks CARRY_ITERS such groups (coalesced loads, the wave carry chained in
+// registers); chunks of CARRY_CHUNK limbs are joined by a per-chunk
+// aggregate and a one-workgroup scan.  Three launches: reduce, scan, apply.
 #include "pg_kernels.h"
 
 namespace pg {
 
-struct GP {
-    uint32_t g, p;
+constexpr int CARRY_WAVES = CARRY_THREADS / 64;
+constexpr int CARRY_ITERS = CARRY_CHUNK / CARRY_THREADS;
+
+struct Group {
+    uint64_t X, Y;   // ballots of (g | p) and g over 64 consecutive limbs
 };
 
-__device__ __forceinline__ void limb_tgp(const CarryLaunch &C, size_t j, uint64_t &t, uint32_t &g,
-                                         uint32_t &p) {
+__device__ __forceinline__ uint64_t limb_t(const CarryLaunch &C, size_t j, uint32_t &g, uint32_t &p) {
+    uint64_t t;
     if (j < C.m) {
         const uint64_t lo = C.slo[j];
         const uint64_t h = j ? C.shi[j - 1] : 0;
@@ -28,79 +34,66 @@ __device__ __forceinline__ void limb_tgp(const CarryLaunch &C, size_t j, uint64_
         g = t < lo;
         p = t == ~0ull;
     } else {
-        t = (j == C.m && j) ? C.shi[j - 1] : 0;   // top carry word of the sums
+        t = (j == C.m && j) ? C.shi[j - 1] : 0;   // top word of the last sum
         g = 0;
         p = 0;
     }
+    return t;
 }
 
-// Thread aggregate over its CARRY_PER_THREAD limbs (low to high).
-__device__ __forceinline__ GP thread_gp(const CarryLaunch &C, size_t base, uint64_t *t, uint32_t *g,
-                                        uint32_t *p) {
-    GP a{0, 1};
-#pragma unroll
-    for (int i = 0; i < CARRY_PER_THREAD; ++i) {
-        limb_tgp(C, base + i, t[i], g[i], p[i]);
-        a.g = g[i] | (p[i] & a.g);
-        a.p &= p[i];
-    }
-    return a;
+// Carry out of a group given carry-in cin (0/1).
+__device__ __forceinline__ uint32_t group_out(const Group &G, uint32_t cin) {
+    const uint64_t s0 = G.X + G.Y;
+    const uint64_t s = s0 + cin;
+    return (uint32_t)((s0 < G.X) | (s < s0));
 }
 
-// Wave-level: carry into each lane given the wave's carry-in, and the wave
-// aggregate.
-__device__ __forceinline__ uint32_t wave_carry(GP a, uint32_t wave_cin, GP *wave_agg) {
-    const uint64_t X = __ballot(a.g | a.p);
-    const uint64_t Y = __ballot(a.g);
-    const uint64_t s0 = X + Y;
-    const uint64_t c0 = s0 < X;
-    const uint64_t s = s0 + wave_cin;
-    const uint64_t c1 = s < s0;
-    const uint64_t carries = s ^ X ^ Y;
-    if (wave_agg) {
-        wave_agg->g = (uint32_t)(c0);            // carry out with cin = 0
-        wave_agg->p = (uint32_t)((X ^ Y) == ~0ull);
-    }
-    (void)c1;
-    const int lane = threadIdx.x & 63;
-    return (uint32_t)((carries >> lane) & 1);
+// (G, P) aggregate of a sequence of groups, as a function of carry-in.
+struct Agg {
+    uint32_t g, p;
+};
+__device__ __forceinline__ Agg group_agg(const Group &G) {
+    return Agg{group_out(G, 0), (uint32_t)((G.X ^ G.Y) == ~0ull)};
+}
+__device__ __forceinline__ Agg compose(Agg lo, Agg hi) {   // lo first, then hi
+    return Agg{hi.g | (hi.p & lo.g), hi.p & lo.p};
 }
 
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_reduce(const CarryLaunch C) {
-    __shared__ GP waves[CARRY_THREADS / 64];
-    const size_t base = (size_t)blockIdx.x * CARRY_CHUNK + (size_t)threadIdx.x * CARRY_PER_THREAD;
-    uint64_t t[CARRY_PER_THREAD];
-    uint32_t g[CARRY_PER_THREAD], p[CARRY_PER_THREAD];
-    GP a = thread_gp(C, base, t, g, p);
-    GP w;
-    wave_carry(a, 0, &w);
-    if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = w;
+    __shared__ Agg waves[CARRY_WAVES];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t base = (size_t)blockIdx.x * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
+    Agg a{0, 1};
+#pragma unroll
+    for (int it = 0; it < CARRY_ITERS; ++it) {
+        uint32_t g, p;
+        limb_t(C, base + 64 * it, g, p);
+        const Group G{__ballot(g | p), __ballot(g)};
+        a = compose(a, group_agg(G));
+    }
+    if (lane == 0) waves[wave] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
-        GP b{0, 1};
-        for (int i = 0; i < CARRY_THREADS / 64; ++i) {
-            b.g = waves[i].g | (waves[i].p & b.g);
-            b.p &= waves[i].p;
-        }
+        Agg b{0, 1};
+        for (int i = 0; i < CARRY_WAVES; ++i) b = compose(b, waves[i]);
         C.agg[blockIdx.x] = (uint8_t)(b.g | (b.p << 1));
     }
 }
 
 // One workgroup: exclusive scan of chunk aggregates into chunk carry-ins.
 __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
-    __shared__ GP waves[16];
+    __shared__ Agg waves[16];
     __shared__ uint32_t wave_in[16];
     const size_t per = (C.nchunks + 1023) / 1024;
     const size_t lo = (size_t)threadIdx.x * per;
-    GP a{0, 1};
+    Agg a{0, 1};
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
         const uint32_t v = C.agg[i];
-        a.g = (v & 1) | ((v >> 1) & a.g);
-        a.p &= (v >> 1);
+        a = compose(a, Agg{v & 1, v >> 1});
     }
-    GP w;
-    wave_carry(a, 0, &w);
-    if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = w;
+    // thread-level (G, P) pairs -> carries via the wave sum trick
+    const Group G{__ballot(a.g | a.p), __ballot(a.g)};
+    if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = group_agg(G);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t x = 0;
@@ -110,7 +103,8 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
         }
     }
     __syncthreads();
-    uint32_t x = wave_carry(a, wave_in[threadIdx.x >> 6], nullptr);
+    const uint64_t sum = G.X + G.Y + wave_in[threadIdx.x >> 6];
+    uint32_t x = (uint32_t)(((sum ^ G.X ^ G.Y) >> (threadIdx.x & 63)) & 1);
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
         C.cin[i] = (uint8_t)x;
         const uint32_t v = C.agg[i];
@@ -119,30 +113,38 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
 }
 
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_apply(const CarryLaunch C) {
-    __shared__ GP waves[CARRY_THREADS / 64];
-    __shared__ uint32_t wave_in[CARRY_THREADS / 64];
-    const size_t base = (size_t)blockIdx.x * CARRY_CHUNK + (size_t)threadIdx.x * CARRY_PER_THREAD;
-    uint64_t t[CARRY_PER_THREAD];
-    uint32_t g[CARRY_PER_THREAD], p[CARRY_PER_THREAD];
-    GP a = thread_gp(C, base, t, g, p);
-    GP w;
-    wave_carry(a, 0, &w);
-    if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = w;
+    __shared__ Agg waves[CARRY_WAVES];
+    __shared__ uint32_t wave_in[CARRY_WAVES];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t base = (size_t)blockIdx.x * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
+    uint64_t t[CARRY_ITERS];
+    Group G[CARRY_ITERS];
+    Agg a{0, 1};
+#pragma unroll
+    for (int it = 0; it < CARRY_ITERS; ++it) {
+        uint32_t g, p;
+        t[it] = limb_t(C, base + 64 * it, g, p);
+        G[it] = Group{__ballot(g | p), __ballot(g)};
+        a = compose(a, group_agg(G[it]));
+    }
+    if (lane == 0) waves[wave] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t x = C.cin[blockIdx.x];
-        for (int i = 0; i < CARRY_THREADS / 64; ++i) {
+        for (int i = 0; i < CARRY_WAVES; ++i) {
             wave_in[i] = x;
             x = waves[i].g | (waves[i].p & x);
         }
     }
     __syncthreads();
-    uint32_t x = wave_carry(a, wave_in[threadIdx.x >> 6], nullptr);
+    uint32_t cin = wave_in[wave];
 #pragma unroll
-    for (int i = 0; i < CARRY_PER_THREAD; ++i) {
-        const size_t j = base + i;
-        if (j < C.out_len) C.out[j] = t[i] + x;
-        x = g[i] | (p[i] & x);
+    for (int it = 0; it < CARRY_ITERS; ++it) {
+        const uint64_t sum = G[it].X + G[it].Y + cin;
+        const uint32_t x = (uint32_t)(((sum ^ G[it].X ^ G[it].Y) >> lane) & 1);
+        const size_t j = base + 64 * it;
+        if (j < C.out_len) C.out[j] = t[it] + x;
+        cin = group_out(G[it], cin);
     }
 }
 
