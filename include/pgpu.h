@@ -75,8 +75,11 @@ int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs);
 uint64_t *pg_buf_ptr(pg_buf *h);
 
 /* r = a * b with all three operands resident in HBM (same contract as
- * pg_mul; r may alias a or b).  Asynchronous unless r_len is non-NULL, in
- * which case the call waits to read the product length. */
+ * pg_mul; r may alias a or b; an, bn are trimmed lengths).  Asynchronous
+ * unless r_len or fft_max_round is non-NULL, in which case the call waits to
+ * read them.  The transform length is planned from the limb counts alone (as
+ * if the top limbs were full), so no device-to-host read precedes the
+ * launches. */
 int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an,
                const pg_buf *b, size_t bn, int path, double *fft_max_round);
 

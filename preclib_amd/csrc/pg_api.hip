@@ -125,12 +125,10 @@ int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an, const pg_bu
     int st = check_operands(an, bn, r->cap);
     if (st) return st;
     Context &C = ctx();
-    uint64_t tops[2];
-    if (hipMemcpyAsync(&tops[0], a->d + an - 1, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
-        hipMemcpyAsync(&tops[1], b->d + bn - 1, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
-        hipStreamSynchronize(C.stream) != hipSuccess)
-        return set_error(PG_EDEVICE, "pg_mul_dev: reading top limbs failed");
-    st = mul_device(r->d, a->d, an, tops[0], b->d, bn, tops[1], path, fft_max_round);
+    // The top limbs stay on the device: plan with full top limbs (~0).  The
+    // transform is then at least as long as the exact digit count needs, so
+    // the product is the same; no host round trip per product.
+    st = mul_device(r->d, a->d, an, ~0ull, b->d, bn, ~0ull, path, fft_max_round);
     if (st) return st;
     if (r_len) {
         uint64_t top = 0;
@@ -191,6 +189,6 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
         if (st != PG_ERANGE) return st;
         path = PG_PATH_FFT;   // beyond the NTT caps the reference multiplies with mul_fft
     }
-    return set_error(PG_ERANGE, "FFT path not available yet");
+    return fft_mul_device(rp, ap, an, atop, bp, bn, btop, fft_max_round);
 }
 }  // namespace pg

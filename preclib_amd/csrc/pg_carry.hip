@@ -25,16 +25,30 @@ struct Group {
     uint64_t X, Y;   // ballots of (g | p) and g over 64 consecutive limbs
 };
 
+// Limb sum j as (lo, hi).  With parts == 2 a limb is the sum of two 128-bit
+// parts whose digits start at bit 0 and bit 32 of the limb.
+__device__ __forceinline__ void limb_sum(const CarryLaunch &C, size_t j, uint64_t &lo, uint64_t &hi) {
+    if (C.parts == 1) {
+        lo = C.slo[j];
+        hi = C.shi[j];
+    } else {
+        const uint64_t l0 = C.slo[2 * j], h0 = C.shi[2 * j];
+        const uint64_t l1 = C.slo[2 * j + 1], h1 = C.shi[2 * j + 1];
+        lo = l0 + (l1 << 32);
+        hi = h0 + (l1 >> 32) + (h1 << 32) + (lo < l0);
+    }
+}
+
 __device__ __forceinline__ uint64_t limb_t(const CarryLaunch &C, size_t j, uint32_t &g, uint32_t &p) {
-    uint64_t t;
+    uint64_t t, lo, hi, hprev = 0;
+    if (j && j <= C.m) limb_sum(C, j - 1, lo, hprev);
     if (j < C.m) {
-        const uint64_t lo = C.slo[j];
-        const uint64_t h = j ? C.shi[j - 1] : 0;
-        t = lo + h;
+        limb_sum(C, j, lo, hi);
+        t = lo + hprev;
         g = t < lo;
         p = t == ~0ull;
     } else {
-        t = (j == C.m && j) ? C.shi[j - 1] : 0;   // top word of the last sum
+        t = j == C.m ? hprev : 0;   // top word of the last sum
         g = 0;
         p = 0;
     }

@@ -27,7 +27,7 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 #endif
 constexpr int ROW_LOG_MAX = PG_ROW_LOG_MAX;
 constexpr int plan_l2(int logn) {
-    return logn <= ROW_LOG_MAX ? logn : cmin(ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 10));
+    return logn <= 11 ? logn : cmin(ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 10));
 }
 constexpr int plan_l1(int logn) { return logn - plan_l2(logn); }
 constexpr int plan_lt(int logn) { return cmin(cmin(6, 14 - plan_l1(logn)), plan_l2(logn)); }
@@ -85,14 +85,50 @@ struct StageHook {
 hipError_t launch_ntt(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
 hipError_t ntt_set_lds_limits();
 
+// ---- FP64 FFT product (pg_fft.hip) -----------------------------------------
+// Complex points n = 2^logn.  Rows of N2 = 2^fft_l2 points live in LDS (one
+// row of complex doubles); columns in tiles of T = 2^fft_lt.
+constexpr int FFT_MIN_LOG = 6, FFT_MAX_LOG = 24, FFT_ROW_LOG_MAX = 13;
+constexpr int fft_l2(int logn) {
+    return logn <= 10 ? logn : cmin(FFT_ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 11));
+}
+constexpr int fft_l1(int logn) { return logn - fft_l2(logn); }
+constexpr int fft_lt(int logn) { return fft_l1(logn) == 0 ? cmin(logn, 6) : cmin(4, 13 - fft_l1(logn)); }
+// radix passes of at most 2^4 points (16 complex doubles in registers)
+constexpr int fsplit_np(int l) { return l ? (l + 3) / 4 : 0; }
+constexpr int fsplit_lr(int l, int i) {
+    return fsplit_np(l) == 0 ? 0 : (i < fsplit_np(l) ? l / fsplit_np(l) + (i < l % fsplit_np(l) ? 1 : 0) : 0);
+}
+constexpr int FFT_THREADS = 512;
+
+struct FftK {
+    double2 w16[8], iw16[8];    // exp(+-2 pi i k / 16), k < 8
+};
+
+struct FftLaunch {
+    const uint64_t *a, *b;      // operand limbs (b may equal a)
+    size_t an, bn;
+    int logn, bits, lobits;     // digit width 16 or 8
+    double2 *spec;              // [n] complex, natural row-major
+    uint64_t *slo, *shi;        // limb sums (or half-limb parts)
+    uint64_t *maxerr;           // bit pattern of the largest rounding distance (atomicMax)
+    FftK k;
+    const double2 *col_tw[MAX_PASS], *col_itw[MAX_PASS];
+    const double2 *row_tw[MAX_PASS], *row_itw[MAX_PASS];
+    const double2 *glo, *ghi;   // omega_n^e = glo[e mod 2^lobits] * ghi[e >> lobits]
+};
+hipError_t launch_fft(const FftLaunch &L, hipStream_t stream, const StageHook &hook);
+hipError_t fft_set_lds_limits();
+
 // ---- carry propagation (pg_carry.hip) ----------------------------------------
 constexpr int CARRY_THREADS = 256;
 constexpr int CARRY_PER_THREAD = 8;
 constexpr int CARRY_CHUNK = CARRY_THREADS * CARRY_PER_THREAD;
 
 struct CarryLaunch {
-    const uint64_t *slo, *shi;  // limb sums, m entries
+    const uint64_t *slo, *shi;  // limb sums, m entries (parts * m when parts == 2)
     size_t m;                   // number of limb sums
+    int parts;                  // 1: one sum per limb; 2: two 32-bit-aligned halves per limb
     uint64_t *out;              // normalized limbs
     size_t out_len;             // limbs to write (>= m is allowed: zero fill)
     uint8_t *agg;               // per-chunk (G | P << 1)

@@ -55,6 +55,14 @@ struct GlobalTw {
     int lobits = 0;
 };
 
+struct FftTw {
+    const double2 *fwd = nullptr, *inv = nullptr;
+};
+struct FftGlobal {
+    const double2 *glo = nullptr, *ghi = nullptr;
+    int lobits = 0;
+};
+
 struct StageStat {
     uint64_t launches = 0;
     double ms = 0.0, bytes = 0.0;
@@ -74,6 +82,10 @@ public:
 
     const TwTables &tw(int prime, int logB, int lr);
     const GlobalTw &gtw(int prime, int logn);
+    const FftTw &ftw(int logB, int lr);
+    const FftGlobal &fgtw(int logn);
+    FftK fk{};
+    uint64_t *maxerr = nullptr;   // device word for the FFT rounding distance
 
     // timing
     bool timing = false;
@@ -94,6 +106,8 @@ public:
 private:
     std::map<long, TwTables> tw_;
     std::map<long, GlobalTw> gtw_;
+    std::map<long, FftTw> ftw_;
+    std::map<long, FftGlobal> fgtw_;
     std::vector<void *> tables_;   // table allocations (freed at shutdown)
     void *table_alloc(size_t bytes);
 };
@@ -105,5 +119,9 @@ int set_error(int code, const std::string &msg);
 // rp must hold an + bn limbs; it may alias ap or bp.  Returns a status code.
 int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
                    const uint64_t *bp, size_t bn, uint64_t btop, int mode);
+// mul_fft on device pointers; *max_round (if non-null) receives the largest
+// rounding distance of this product (waits for the device).
+int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
+                   const uint64_t *bp, size_t bn, uint64_t btop, double *max_round);
 
 }  // namespace pg

@@ -138,6 +138,16 @@ int Context::init(int dev, size_t budget) {
     crt.inv123q = shoup_q(crt.inv123, PRIME[2]);
     for (int k = 0; k < 3; ++k) crt.red[k] = (uint32_t)((((uint64_t)1) << 32) / PRIME[k]);
     HIP_TRY(ntt_set_lds_limits());
+    HIP_TRY(fft_set_lds_limits());
+    HIP_TRY(hipMalloc(&maxerr, 256));
+    {
+        const long double PI = 3.141592653589793238462643383279502884L;
+        for (int k = 0; k < 8; ++k) {
+            const long double a = 2.0L * PI * k / 16.0L;
+            fk.w16[k] = make_double2((double)cosl(a), (double)sinl(a));
+            fk.iw16[k] = make_double2((double)cosl(a), -(double)sinl(a));
+        }
+    }
     device = dev;
     return PG_OK;
 }
@@ -153,6 +163,10 @@ void Context::shutdown() {
     tables_.clear();
     tw_.clear();
     gtw_.clear();
+    ftw_.clear();
+    fgtw_.clear();
+    if (maxerr) hipFree(maxerr);
+    maxerr = nullptr;
     pool.release_all();
     hipStreamDestroy(stream);
     stream = nullptr;
@@ -246,6 +260,65 @@ const GlobalTw &Context::gtw(int prime, int logn) {
     return gtw_[key] = g;
 }
 
+// FFT pass table [k1][j2] = exp(+-2 pi i j2 k1 / B), from long double.
+const FftTw &Context::ftw(int logB, int lr) {
+    const long key = ((long)logB << 8) | lr;
+    auto it = ftw_.find(key);
+    if (it != ftw_.end()) return it->second;
+    FftTw t;
+    const int R = 1 << lr, M = 1 << (logB - lr);
+    if (M > 1) {
+        const long double PI = 3.141592653589793238462643383279502884L;
+        const long double B = (long double)(1L << logB);
+        std::vector<double2> f((size_t)R * M), v((size_t)R * M);
+        for (int k1 = 0; k1 < R; ++k1)
+            for (int j2 = 0; j2 < M; ++j2) {
+                const long double a = 2.0L * PI * (long double)((long)j2 * k1) / B;
+                const double c = (double)cosl(a), sn = (double)sinl(a);
+                f[(size_t)k1 * M + j2] = make_double2(c, sn);
+                v[(size_t)k1 * M + j2] = make_double2(c, -sn);
+            }
+        double2 *df = (double2 *)table_alloc(f.size() * sizeof(double2));
+        double2 *dv = (double2 *)table_alloc(v.size() * sizeof(double2));
+        if (df && dv) {
+            hipMemcpy(df, f.data(), f.size() * sizeof(double2), hipMemcpyHostToDevice);
+            hipMemcpy(dv, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice);
+        }
+        t.fwd = df;
+        t.inv = dv;
+    }
+    return ftw_[key] = t;
+}
+
+// omega_n^e = glo[e mod 2^s] * ghi[e >> s], omega_n = exp(+2 pi i / n).
+const FftGlobal &Context::fgtw(int logn) {
+    auto it = fgtw_.find(logn);
+    if (it != fgtw_.end()) return it->second;
+    FftGlobal g;
+    const int s = (logn + 1) / 2;
+    const size_t nlo = (size_t)1 << s, nhi = (size_t)1 << (logn - s);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const long double n = (long double)(1L << logn);
+    std::vector<double2> lo(nlo), hi(nhi);
+    for (size_t e = 0; e < nlo; ++e) {
+        const long double a = 2.0L * PI * (long double)e / n;
+        lo[e] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    for (size_t e = 0; e < nhi; ++e) {
+        const long double a = 2.0L * PI * (long double)(e * nlo) / n;
+        hi[e] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    double2 *d = (double2 *)table_alloc((nlo + nhi) * sizeof(double2));
+    if (d) {
+        hipMemcpy(d, lo.data(), nlo * sizeof(double2), hipMemcpyHostToDevice);
+        hipMemcpy(d + nlo, hi.data(), nhi * sizeof(double2), hipMemcpyHostToDevice);
+        g.glo = d;
+        g.ghi = d + nlo;
+    }
+    g.lobits = s;
+    return fgtw_[logn] = g;
+}
+
 // ---- timing -------------------------------------------------------------------
 
 void Context::stage_begin(hipEvent_t &a) {
@@ -321,9 +394,10 @@ struct Scratch {
 };
 
 int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *out, size_t out_len,
-                 Scratch &s) {
+                 Scratch &s, int parts = 1) {
     Context &C = ctx();
     CarryLaunch cl{};
+    cl.parts = parts;
     cl.slo = slo;
     cl.shi = shi;
     cl.m = m;
@@ -397,6 +471,72 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     if ((l1 && !L.spec) || !L.prod || !L.slo || !L.shi) return set_error(PG_ENOMEM, "NTT scratch");
     HIP_TRY(launch_ntt(L, C.stream, C.hook()));
     return launch_carry(L.slo, L.shi, m, rp, an + bn, s);
+}
+
+}  // namespace pg
+
+namespace pg {
+
+// mul_fft_bits (src/mul_fft.cpp:340-384) on device pointers.
+int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp, size_t bn,
+                   uint64_t btop, double *max_round) {
+    Context &C = ctx();
+    if (!C.stream) return set_error(PG_ENOINIT, "pg_init not called");
+    if (!an || !bn || !atop || !btop) return set_error(PG_EINVAL, "operands must be trimmed and nonzero");
+    const int bits = std::max(an, bn) <= 1024 ? 16 : 8;   // mul_fft.cpp:390-393
+    const int dpl = 64 / bits;
+    auto count = [&](size_t xn, uint64_t top) {            // fft_digit_count, mul_fft.cpp:222-232
+        size_t d = (xn - 1) * dpl;
+        do { ++d; top >>= bits; } while (top);
+        return d;
+    };
+    const size_t da = count(an, atop), db = count(bn, btop);
+    int logn = 0;
+    while (((size_t)1 << logn) < da + db) ++logn;
+    if (logn < FFT_MIN_LOG) logn = FFT_MIN_LOG;
+    if (logn > FFT_MAX_LOG) return set_error(PG_ERANGE, "FFT limit 2^24 complex points");
+    const size_t n = (size_t)1 << logn;
+    const int l1 = fft_l1(logn), l2 = fft_l2(logn), lt = fft_lt(logn);
+    const int seg = std::min(1 << lt, dpl);     // digits per limb sum
+    const int parts = dpl / seg;
+    Scratch s;
+    FftLaunch L{};
+    L.a = ap; L.b = bp; L.an = an; L.bn = bn;
+    L.logn = logn; L.bits = bits;
+    L.k = C.fk;
+    L.maxerr = C.maxerr;
+    int logB = l1;
+    for (int i = 0; i < fsplit_np(l1); ++i) {
+        const FftTw &t = C.ftw(logB, fsplit_lr(l1, i));
+        L.col_tw[i] = t.fwd; L.col_itw[i] = t.inv;
+        logB -= fsplit_lr(l1, i);
+    }
+    logB = l2;
+    for (int i = 0; i < fsplit_np(l2); ++i) {
+        const FftTw &t = C.ftw(logB, fsplit_lr(l2, i));
+        L.row_tw[i] = t.fwd; L.row_itw[i] = t.inv;
+        logB -= fsplit_lr(l2, i);
+    }
+    const FftGlobal &g = C.fgtw(logn);
+    if (!g.glo) return set_error(PG_ENOMEM, "FFT twiddle tables");
+    L.glo = g.glo; L.ghi = g.ghi; L.lobits = g.lobits;
+    L.spec = s.get<double2>(n);
+    const size_t nsum = n / seg;
+    L.slo = s.get<uint64_t>(nsum);
+    L.shi = s.get<uint64_t>(nsum);
+    if (!L.spec || !L.slo || !L.shi) return set_error(PG_ENOMEM, "FFT scratch");
+    HIP_TRY(launch_fft(L, C.stream, C.hook()));
+    int st = launch_carry(L.slo, L.shi, n / dpl, rp, an + bn, s, parts);
+    if (st) return st;
+    if (max_round) {
+        uint64_t bitsv = 0;
+        HIP_TRY(hipMemcpyAsync(&bitsv, C.maxerr, 8, hipMemcpyDeviceToHost, C.stream));
+        HIP_TRY(hipStreamSynchronize(C.stream));
+        double e;
+        std::memcpy(&e, &bitsv, 8);
+        *max_round = e;
+    }
+    return PG_OK;
 }
 
 }  // namespace pg
