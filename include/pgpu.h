@@ -83,6 +83,41 @@ uint64_t *pg_buf_ptr(pg_buf *h);
 int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an,
                const pg_buf *b, size_t bn, int path, double *fft_max_round);
 
+/*
+ * Many products in one submission.  preclib has no batch interface: its
+ * callers multiply one pair at a time through mul_into
+ * (src/mul_basic.cpp:238-276), and every job here is decomposed exactly as
+ * mul_into would decompose it (small / blocked / NTT / FFT, same thresholds).
+ * Products of the same transform length run in the same launch.
+ */
+typedef struct {
+    const uint64_t *a;      /* operand limbs (trimmed)                      */
+    size_t an;
+    const uint64_t *b;
+    size_t bn;
+    uint64_t *r;            /* product, capacity r_cap >= an + bn           */
+    size_t r_cap;
+    size_t r_len;           /* out: trimmed product length                  */
+    int path;               /* PG_PATH_*                                    */
+    int status;             /* out: 0 or a PG_E* code                       */
+    double fft_max_round;   /* out: largest FFT rounding distance, 0 if none */
+} pg_job;
+
+/* Host buffers in and out: uploads every operand, runs the batch, downloads
+ * every product.  Returns 0 if every job succeeded. */
+int pg_mul_batch(pg_job *jobs, size_t n);
+
+/* Device-resident batches: a, b and r of every job are device pointers
+ * (pg_buf_ptr).  pg_batch_plan decomposes and allocates once; pg_batch_run
+ * enqueues the whole batch (captured into a hipGraph on its first run and
+ * replayed afterwards); results are in place after pg_sync.  r_len is not
+ * filled (the caller knows an + bn; the top limb may be zero). */
+typedef struct pg_batch pg_batch;
+int pg_batch_plan(pg_batch **out, const pg_job *jobs, size_t n);
+int pg_batch_run(pg_batch *b);
+int pg_batch_errors(pg_batch *b, double *fft_max_round, size_t n);   /* per job; waits */
+int pg_batch_free(pg_batch *b);
+
 /* Waits for all queued device work. */
 int pg_sync(void);
 

@@ -1,6 +1,7 @@
 // pg_api.hip -- extern "C" entry points of libpgpu.so (declared in pgpu.h).
 #include <algorithm>
 #include <cstring>
+#include "pg_batch.h"
 #include "pg_runtime.h"
 #include "pgpu.h"
 
@@ -80,7 +81,6 @@ int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs) {
 static int check_operands(size_t an, size_t bn, size_t r_cap) {
     if (!an || !bn) return set_error(PG_EINVAL, "operands must have at least one limb");
     if (r_cap < an + bn) return set_error(PG_EINVAL, "r_cap < an + bn");
-    if (std::max(an, bn) <= 192) return set_error(PG_ERANGE, "operands of <= 192 limbs take the host basecase");
     return PG_OK;
 }
 
@@ -140,6 +140,105 @@ int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an, const pg_bu
     return PG_OK;
 }
 
+struct pg_batch {
+    pg::Batch b;
+};
+
+int pg_batch_plan(pg_batch **out, const pg_job *jobs, size_t n) {
+    LOCK;
+    NEED_INIT;
+    if (!out || (!jobs && n)) return set_error(PG_EINVAL, "pg_batch_plan: bad argument");
+    pg_batch *h = new pg_batch;
+    const int st = h->b.build(jobs, n);
+    if (st) {
+        delete h;
+        return st;
+    }
+    *out = h;
+    return PG_OK;
+}
+
+int pg_batch_run(pg_batch *h) {
+    LOCK;
+    NEED_INIT;
+    if (!h) return set_error(PG_EINVAL, "pg_batch_run: null handle");
+    return h->b.run();
+}
+
+int pg_batch_errors(pg_batch *h, double *err, size_t n) {
+    LOCK;
+    NEED_INIT;
+    if (!h || (!err && n)) return set_error(PG_EINVAL, "pg_batch_errors: bad argument");
+    std::vector<double> e;
+    const int st = h->b.read_errors(e);
+    if (st) return st;
+    for (size_t i = 0; i < n && i < e.size(); ++i) err[i] = e[i];
+    return PG_OK;
+}
+
+int pg_batch_free(pg_batch *h) {
+    LOCK;
+    if (h) {
+        if (ctx().stream) hipStreamSynchronize(ctx().stream);
+        delete h;
+    }
+    return PG_OK;
+}
+
+int pg_mul_batch(pg_job *jobs, size_t n) {
+    LOCK;
+    NEED_INIT;
+    if (!jobs && n) return set_error(PG_EINVAL, "pg_mul_batch: null jobs");
+    Context &C = ctx();
+    size_t total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        jobs[i].status = PG_OK;
+        jobs[i].r_len = 0;
+        jobs[i].fft_max_round = 0.0;
+        if (!jobs[i].a || !jobs[i].b || !jobs[i].r || !jobs[i].an || !jobs[i].bn ||
+            jobs[i].r_cap < jobs[i].an + jobs[i].bn)
+            return set_error(PG_EINVAL, "pg_mul_batch: bad job " + std::to_string(i));
+        total += 2 * (jobs[i].an + jobs[i].bn);
+    }
+    uint64_t *dev = (uint64_t *)C.pool.get(total * 8 + 8);
+    if (!dev) return set_error(PG_ENOMEM, "pg_mul_batch: device budget exhausted");
+    std::vector<pg_job> dj(jobs, jobs + n);
+    size_t off = 0;
+    int st = PG_OK;
+    for (size_t i = 0; i < n && !st; ++i) {
+        uint64_t *da = dev + off, *db = da + jobs[i].an, *dr = db + jobs[i].bn;
+        if (hipMemcpyAsync(da, jobs[i].a, jobs[i].an * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess ||
+            hipMemcpyAsync(db, jobs[i].b, jobs[i].bn * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess)
+            st = set_error(PG_EDEVICE, "pg_mul_batch: upload failed");
+        dj[i].a = da; dj[i].b = db; dj[i].r = dr;
+        dj[i].r_cap = jobs[i].an + jobs[i].bn;
+        off += 2 * (jobs[i].an + jobs[i].bn);
+    }
+    std::vector<double> errs;
+    if (!st) {
+        pg::Batch b;
+        b.use_graph = false;
+        st = b.build(dj.data(), n);
+        if (!st) st = b.run();
+        if (!st) st = b.read_errors(errs);
+        for (size_t i = 0; i < n && !st; ++i)
+            if (hipMemcpyAsync(jobs[i].r, dj[i].r, (jobs[i].an + jobs[i].bn) * 8, hipMemcpyDeviceToHost,
+                               C.stream) != hipSuccess)
+                st = set_error(PG_EDEVICE, "pg_mul_batch: download failed");
+        if (hipStreamSynchronize(C.stream) != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul_batch: device error");
+    }
+    C.pool.put(dev);
+    for (size_t i = 0; i < n; ++i) {
+        jobs[i].status = st;
+        if (st) continue;
+        size_t len = jobs[i].an + jobs[i].bn;
+        while (len && jobs[i].r[len - 1] == 0) --len;
+        jobs[i].r_len = len;
+        jobs[i].fft_max_round = i < errs.size() ? errs[i] : 0.0;
+    }
+    return st;
+}
+
 int pg_timing_enable(int on) {
     LOCK;
     ctx().timing = on != 0;
@@ -180,9 +279,29 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
                uint64_t btop, int path, double *fft_max_round) {
     if (fft_max_round) *fft_max_round = 0.0;
     const size_t lo = std::min(an, bn), hi = std::max(an, bn);
-    if (path == PG_PATH_AUTO) {
-        if (hi > 2 * lo) return set_error(PG_ERANGE, "blocked path not available yet");
-        path = hi > 6144 ? PG_PATH_NTT : PG_PATH_FFT;
+    const bool sqr = ap == bp && an == bn;
+    // Below the transform threshold mul_ntt / mul_fft fall back to mul_basic
+    // (mul_ntt.cpp:496, mul_fft.cpp:388); mul_into takes the schoolbook for a
+    // side <= 24 limbs and the blocked multiply for hi > 2 lo.
+    const bool small = hi <= 192 || (path == PG_PATH_AUTO && lo <= 24);
+    if (small || path == PG_PATH_AUTO) {
+        if (small || (!sqr && hi > 2 * lo)) {
+            // small leaf / mul_unbalanced_into (mul_basic.cpp:207-236) as a batch of one
+            pg_job j{};
+            j.a = ap; j.an = an; j.b = bp; j.bn = bn; j.r = rp; j.r_cap = an + bn; j.path = PG_PATH_AUTO;
+            Batch b;
+            b.use_graph = false;
+            int st = b.build(&j, 1);
+            if (!st) st = b.run();
+            if (!st && fft_max_round) {
+                std::vector<double> e;
+                st = b.read_errors(e);
+                if (!st) *fft_max_round = e[0];
+            }
+            if (!st && hipStreamSynchronize(ctx().stream) != hipSuccess) st = set_error(PG_EDEVICE, "device error");
+            return st;
+        }
+        path = (!sqr && hi > 6144) ? PG_PATH_NTT : PG_PATH_FFT;
     }
     if (path == PG_PATH_NTT || path == PG_PATH_NTT_REF) {
         int st = ntt_mul_device(rp, ap, an, atop, bp, bn, btop, path == PG_PATH_NTT_REF ? MODE_REF : MODE_WIDE);

@@ -25,30 +25,51 @@ struct Group {
     uint64_t X, Y;   // ballots of (g | p) and g over 64 consecutive limbs
 };
 
-// Limb sum j as (lo, hi).  With parts == 2 a limb is the sum of two 128-bit
-// parts whose digits start at bit 0 and bit 32 of the limb.
-__device__ __forceinline__ void limb_sum(const CarryLaunch &C, size_t j, uint64_t &lo, uint64_t &hi) {
-    if (C.parts == 1) {
-        lo = C.slo[j];
-        hi = C.shi[j];
-    } else {
-        const uint64_t l0 = C.slo[2 * j], h0 = C.shi[2 * j];
-        const uint64_t l1 = C.slo[2 * j + 1], h1 = C.shi[2 * j + 1];
-        lo = l0 + (l1 << 32);
-        hi = h0 + (l1 >> 32) + (h1 << 32) + (lo < l0);
+__device__ __forceinline__ const CarryNode &node_of(const CarryLaunch &C, size_t chunk) {
+    return C.nodes ? C.nodes[C.chunk_node[chunk]] : C.one;
+}
+
+// Limb sum j of a node as (lo, hi).
+__device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t &lo, uint64_t &hi) {
+    if (N.kind == NODE_PLAIN) {
+        if (N.parts == 1) {
+            lo = N.slo[j];
+            hi = N.shi[j];
+        } else {   // two parts starting at bit 0 and bit 32 of the limb
+            const uint64_t l0 = N.slo[2 * j], h0 = N.shi[2 * j];
+            const uint64_t l1 = N.slo[2 * j + 1], h1 = N.shi[2 * j + 1];
+            lo = l0 + (l1 << 32);
+            hi = h0 + (l1 >> 32) + (h1 << 32) + (lo < l0);
+        }
+        return;
+    }
+    lo = 0;
+    hi = 0;
+    const size_t k = j / N.B;
+    for (int d = 0; d < 2; ++d) {
+        if (k < (size_t)d) break;
+        const size_t kb = k - d;
+        if (kb >= (size_t)N.nblocks) continue;
+        const BlockRef &blk = N.blocks[kb];
+        const size_t local = j - kb * N.B;
+        if (local < blk.len) {
+            const uint64_t s = lo + blk.slo[local];
+            hi += blk.shi[local] + (s < lo);
+            lo = s;
+        }
     }
 }
 
-__device__ __forceinline__ uint64_t limb_t(const CarryLaunch &C, size_t j, uint32_t &g, uint32_t &p) {
+__device__ __forceinline__ uint64_t limb_t(const CarryNode &N, size_t j, uint32_t &g, uint32_t &p) {
     uint64_t t, lo, hi, hprev = 0;
-    if (j && j <= C.m) limb_sum(C, j - 1, lo, hprev);
-    if (j < C.m) {
-        limb_sum(C, j, lo, hi);
+    if (j && j <= N.m) limb_sum(N, j - 1, lo, hprev);
+    if (j < N.m) {
+        limb_sum(N, j, lo, hi);
         t = lo + hprev;
         g = t < lo;
         p = t == ~0ull;
     } else {
-        t = j == C.m ? hprev : 0;   // top word of the last sum
+        t = j == N.m ? hprev : 0;   // top word of the last sum
         g = 0;
         p = 0;
     }
@@ -75,13 +96,14 @@ __device__ __forceinline__ Agg compose(Agg lo, Agg hi) {   // lo first, then hi
 
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_reduce(const CarryLaunch C) {
     __shared__ Agg waves[CARRY_WAVES];
+    const CarryNode &N = node_of(C, blockIdx.x);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t base = (size_t)blockIdx.x * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
+    const size_t base = (blockIdx.x - N.chunk0) * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
     Agg a{0, 1};
 #pragma unroll
     for (int it = 0; it < CARRY_ITERS; ++it) {
         uint32_t g, p;
-        limb_t(C, base + 64 * it, g, p);
+        limb_t(N, base + 64 * it, g, p);
         const Group G{__ballot(g | p), __ballot(g)};
         a = compose(a, group_agg(G));
     }
@@ -95,6 +117,11 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_reduce(const CarryLaunc
 }
 
 // One workgroup: exclusive scan of chunk aggregates into chunk carry-ins.
+// The first chunk of every node starts with carry 0 (a "kill" before it).
+__device__ __forceinline__ bool node_start(const CarryLaunch &C, size_t i) {
+    return C.nodes ? C.nodes[C.chunk_node[i]].chunk0 == i : i == 0;
+}
+
 __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     __shared__ Agg waves[16];
     __shared__ uint32_t wave_in[16];
@@ -103,6 +130,7 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     Agg a{0, 1};
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
         const uint32_t v = C.agg[i];
+        if (node_start(C, i)) a = Agg{0, 0};
         a = compose(a, Agg{v & 1, v >> 1});
     }
     // thread-level (G, P) pairs -> carries via the wave sum trick
@@ -120,6 +148,7 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     const uint64_t sum = G.X + G.Y + wave_in[threadIdx.x >> 6];
     uint32_t x = (uint32_t)(((sum ^ G.X ^ G.Y) >> (threadIdx.x & 63)) & 1);
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
+        if (node_start(C, i)) x = 0;
         C.cin[i] = (uint8_t)x;
         const uint32_t v = C.agg[i];
         x = (v & 1) | ((v >> 1) & x);
@@ -129,15 +158,16 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_apply(const CarryLaunch C) {
     __shared__ Agg waves[CARRY_WAVES];
     __shared__ uint32_t wave_in[CARRY_WAVES];
+    const CarryNode &N = node_of(C, blockIdx.x);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t base = (size_t)blockIdx.x * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
+    const size_t base = (blockIdx.x - N.chunk0) * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
     uint64_t t[CARRY_ITERS];
     Group G[CARRY_ITERS];
     Agg a{0, 1};
 #pragma unroll
     for (int it = 0; it < CARRY_ITERS; ++it) {
         uint32_t g, p;
-        t[it] = limb_t(C, base + 64 * it, g, p);
+        t[it] = limb_t(N, base + 64 * it, g, p);
         G[it] = Group{__ballot(g | p), __ballot(g)};
         a = compose(a, group_agg(G[it]));
     }
@@ -157,7 +187,10 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_apply(const CarryLaunch
         const uint64_t sum = G[it].X + G[it].Y + cin;
         const uint32_t x = (uint32_t)(((sum ^ G[it].X ^ G[it].Y) >> lane) & 1);
         const size_t j = base + 64 * it;
-        if (j < C.out_len) C.out[j] = t[it] + x;
+        if (j < N.out_len) {
+            N.out[j] = t[it] + x;
+            if (N.out_hi) N.out_hi[j] = 0;
+        }
         cin = group_out(G[it], cin);
     }
 }

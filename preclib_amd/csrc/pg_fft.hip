@@ -193,12 +193,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_col_fwd(const FftLaunch L) 
     constexpr int N1 = 1 << L1, T = 1 << LT, NE = N1 * T;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     extern __shared__ double2 clds[];
+    const FftLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const int tile = fxcd_tile(blockIdx.x, gridDim.x);
     const uint32_t c0 = (uint32_t)tile << LT;
     for (int e = threadIdx.x; e < NE; e += FFT_THREADS) {
         const int j1 = e >> LT, cc = e & (T - 1);
         const size_t j = ((size_t)j1 << L2) + c0 + cc;
-        clds[fp(e)] = make_double2(digit_of(L.a, L.an, j, L.bits), digit_of(L.b, L.bn, j, L.bits));
+        clds[fp(e)] = make_double2(digit_of(D.a, D.an, j, L.bits), digit_of(D.b, D.bn, j, L.bits));
     }
     __syncthreads();
     cfwd_from<L1, LT, FFT_THREADS, L1, 0>(clds, L.col_tw, L.k);
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_col_fwd(const FftLaunch L) 
         const uint32_t ex = ((c0 + cc) * cfreq<L1>(rho)) & NMASK;
         double2 x = clds[fp(e)];
         if (ex) x = cmul(x, gtw(L, ex));
-        L.spec[((size_t)rho << L2) + c0 + cc] = x;
+        D.spec[((size_t)rho << L2) + c0 + cc] = x;
     }
 }
 
@@ -221,21 +222,21 @@ __device__ __forceinline__ double2 pair_mul(double2 x, double2 y) {
 }
 
 template <int LOGN>
-__device__ __forceinline__ void load_crow(double2 *lds, const FftLaunch &L, int rho) {
+__device__ __forceinline__ void load_crow(double2 *lds, const FftLaunch &L, const FftLeaf &D, int rho) {
     constexpr int L1 = fft_l1(LOGN), L2 = fft_l2(LOGN), N2 = 1 << L2;
     if constexpr (L1 == 0) {
         for (int j = threadIdx.x; j < N2; j += FFT_THREADS)
-            lds[fp(j)] = make_double2(digit_of(L.a, L.an, j, L.bits), digit_of(L.b, L.bn, j, L.bits));
+            lds[fp(j)] = make_double2(digit_of(D.a, D.an, j, L.bits), digit_of(D.b, D.bn, j, L.bits));
     } else {
-        const double2 *in = L.spec + ((size_t)rho << L2);
+        const double2 *in = D.spec + ((size_t)rho << L2);
         for (int j = threadIdx.x; j < N2; j += FFT_THREADS) lds[fp(j)] = in[j];
     }
 }
 
 template <int LOGN>
-__device__ __forceinline__ void store_crow(const double2 *lds, const FftLaunch &L, int rho) {
+__device__ __forceinline__ void store_crow(const double2 *lds, const FftLeaf &D, int rho) {
     constexpr int L2 = fft_l2(LOGN), N2 = 1 << L2;
-    double2 *out = L.spec + ((size_t)rho << L2);
+    double2 *out = D.spec + ((size_t)rho << L2);
     for (int j = threadIdx.x; j < N2; j += FFT_THREADS) out[j] = lds[fp(j)];
 }
 
@@ -244,7 +245,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_row(const FftLaunch L) {
     constexpr int L1 = fft_l1(LOGN), L2 = fft_l2(LOGN), N1 = 1 << L1, N2 = 1 << L2;
     constexpr int E = (N2 + FFT_THREADS - 1) / FFT_THREADS;
     extern __shared__ double2 clds[];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *L.maxerr = 0;   // read only by k_fft_col_inv
+    const FftLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *D.maxerr = 0;   // read only by k_fft_col_inv
     const uint32_t k1 = blockIdx.x;                 // 0 .. N1/2
     const uint32_t k1p = (N1 - k1) & (N1 - 1);      // partner row frequency
     const int rho = L1 ? (int)cpos<L1>(k1) : 0;
@@ -255,7 +257,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_row(const FftLaunch L) {
     };
     if (k1 == k1p) {
         // self-paired row (k1 = 0 or N1/2): the whole pair convolution is in LDS
-        load_crow<LOGN>(clds, L, rho);
+        load_crow<LOGN>(clds, L, D, rho);
         __syncthreads();
         cfwd_from<L2, 0, FFT_THREADS, L2, 0>(clds, L.row_tw, L.k);
         for (int i = threadIdx.x; i < N2; i += FFT_THREADS) {
@@ -268,12 +270,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_row(const FftLaunch L) {
         }
         __syncthreads();
         cinv_from<L2, 0, FFT_THREADS, L2, 0>(clds, L.row_itw, L.k);
-        store_crow<LOGN>(clds, L, rho);
+        store_crow<LOGN>(clds, D, rho);
         return;
     }
     // row k1 -> registers, row k1p in LDS
     double2 keep[E];
-    load_crow<LOGN>(clds, L, rho);
+    load_crow<LOGN>(clds, L, D, rho);
     __syncthreads();
     cfwd_from<L2, 0, FFT_THREADS, L2, 0>(clds, L.row_tw, L.k);
 #pragma unroll
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_row(const FftLaunch L) {
         if (i < N2) keep[q] = clds[fp(i)];
     }
     __syncthreads();
-    load_crow<LOGN>(clds, L, rhop);
+    load_crow<LOGN>(clds, L, D, rhop);
     __syncthreads();
     cfwd_from<L2, 0, FFT_THREADS, L2, 0>(clds, L.row_tw, L.k);
 #pragma unroll
@@ -297,7 +299,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_row(const FftLaunch L) {
     }
     __syncthreads();
     cinv_from<L2, 0, FFT_THREADS, L2, 0>(clds, L.row_itw, L.k);
-    store_crow<LOGN>(clds, L, rhop);
+    store_crow<LOGN>(clds, D, rhop);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -306,7 +308,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_row(const FftLaunch L) {
     }
     __syncthreads();
     cinv_from<L2, 0, FFT_THREADS, L2, 0>(clds, L.row_itw, L.k);
-    store_crow<LOGN>(clds, L, rho);
+    store_crow<LOGN>(clds, D, rho);
 }
 
 // ---- kernel 3: inverse column pass, rounding, limb sums --------------------------
@@ -327,12 +329,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_col_inv(const FftLaunch L) 
     constexpr int N1 = 1 << L1, T = 1 << LT, NE = N1 * T;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     extern __shared__ double2 clds[];
+    const FftLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const int tile = fxcd_tile(blockIdx.x, gridDim.x);
     const uint32_t c0 = (uint32_t)tile << LT;
     for (int e = threadIdx.x; e < NE; e += FFT_THREADS) {
         const int rho = e >> LT, cc = e & (T - 1);
         const uint32_t ex = ((c0 + cc) * cfreq<L1>(rho)) & NMASK;
-        double2 x = L.spec[((size_t)rho << L2) + c0 + cc];
+        double2 x = D.spec[((size_t)rho << L2) + c0 + cc];
         if (ex) {
             const double2 w = gtw(L, ex);
             x = cmul(x, make_double2(w.x, -w.y));   // omega_n^-e
@@ -359,7 +362,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_col_inv(const FftLaunch L) 
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < FFT_THREADS / 64; ++w) emax = fmax(emax, wmax[w]);
-        atomicMax((unsigned long long *)L.maxerr, (unsigned long long)__double_as_longlong(emax));
+        atomicMax((unsigned long long *)D.maxerr, (unsigned long long)__double_as_longlong(emax));
     }
     // limb sums: DPL digits per limb; a tile row holds T digits, so one sum
     // covers min(T, DPL) digits.  PARTS = 2 when a limb spans two tiles.
@@ -374,8 +377,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_col_inv(const FftLaunch L) 
         for (int t = 0; t < seg; ++t) sacc128(lo, hi, coef[2 * fp(e0 + t)], bits * t);
         const size_t d0 = ((size_t)rho << L2) + c0 + cc;   // first digit of the sum
         const size_t idx = parts == 1 ? d0 / dpl : d0 / seg;
-        L.slo[idx] = lo;
-        L.shi[idx] = hi;
+        if (idx < D.sum_len) {
+            D.slo[idx] = lo;
+            D.shi[idx] = hi;
+        }
     }
 }
 
@@ -386,24 +391,25 @@ static hipError_t launch_fft_size(const FftLaunch &L, hipStream_t stream, const 
     constexpr int L1 = fft_l1(LOGN), L2 = fft_l2(LOGN), LT = fft_lt(LOGN);
     const int ntiles = (1 << L2) >> LT;
     const size_t col_lds = lds_words((size_t)1 << (L1 + LT)) * 16, row_lds = lds_words((size_t)1 << L2) * 16;
-    const double n = (double)(1u << LOGN), io = (double)(L.an + L.bn) * 8.0;
+    const int nl = L.leaves ? L.nleaves : 1;
+    const double n = (double)(1u << LOGN) * nl, io = L.io_bytes;
     if constexpr (L1 > 0) {
         h.begin(h.ctx, "fft_col_fwd");
-        hipLaunchKernelGGL(k_fft_col_fwd<LOGN>, dim3(ntiles), dim3(FFT_THREADS), col_lds, stream, L);
+        hipLaunchKernelGGL(k_fft_col_fwd<LOGN>, dim3(ntiles, 1, nl), dim3(FFT_THREADS), col_lds, stream, L);
         h.end(h.ctx, "fft_col_fwd", io + 32.0 * n);
     }
     h.begin(h.ctx, "fft_row");
-    hipLaunchKernelGGL(k_fft_row<LOGN>, dim3((1 << L1) / 2 + 1), dim3(FFT_THREADS), row_lds, stream, L);
+    hipLaunchKernelGGL(k_fft_row<LOGN>, dim3((1 << L1) / 2 + 1, 1, nl), dim3(FFT_THREADS), row_lds, stream, L);
     h.end(h.ctx, "fft_row", 64.0 * n);
     h.begin(h.ctx, "fft_col_inv");
-    hipLaunchKernelGGL(k_fft_col_inv<LOGN>, dim3(ntiles), dim3(FFT_THREADS), col_lds, stream, L);
+    hipLaunchKernelGGL(k_fft_col_inv<LOGN>, dim3(ntiles, 1, nl), dim3(FFT_THREADS), col_lds, stream, L);
     h.end(h.ctx, "fft_col_inv", 32.0 * n);
     return hipGetLastError();
 }
 
 template <int LOGN>
 static hipError_t fft_limits() {
-    constexpr int lim = 160 * 1024;
+    constexpr int lim = 156 * 1024;   // dynamic LDS limit; leaves room for static LDS
     hipError_t e = hipSuccess;
     if (fft_l1(LOGN) > 0)
         e = hipFuncSetAttribute((const void *)k_fft_col_fwd<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);

@@ -61,12 +61,23 @@ struct CrtK {
     uint32_t red[3];                  // floor(2^32 / p_k): Shoup constants of x * 1 mod p_k
 };
 
-struct NttLaunch {
+// One product ("leaf") of an NTT launch.
+struct NttLeaf {
     const uint64_t *a, *b;      // operand limbs (device)
     size_t an, bn;              // limb counts
     uint32_t *spec;             // forward spectra [2 operands][nprimes][n]
     uint32_t *prod;             // product spectra / residues [nprimes][n]
-    uint64_t *slo, *shi;        // 128-bit limb sums [number of product limbs]
+    uint64_t *slo, *shi;        // 128-bit limb sums
+    size_t sum_len;             // sums written: limbs [0, sum_len)
+};
+
+// A launch covers one leaf (`one`) or, when `leaves` is set, nleaves leaves of
+// the same transform length; blockIdx.z selects the leaf.
+struct NttLaunch {
+    NttLeaf one;
+    const NttLeaf *leaves;
+    int nleaves;
+    double io_bytes;            // operand + product limb bytes of all leaves (timing model)
     int logn, lobits, nprimes, mode;
     PrimeK pk[3];
     CrtK crt;
@@ -105,13 +116,21 @@ struct FftK {
     double2 w16[8], iw16[8];    // exp(+-2 pi i k / 16), k < 8
 };
 
-struct FftLaunch {
+struct FftLeaf {
     const uint64_t *a, *b;      // operand limbs (b may equal a)
     size_t an, bn;
-    int logn, bits, lobits;     // digit width 16 or 8
     double2 *spec;              // [n] complex, natural row-major
     uint64_t *slo, *shi;        // limb sums (or half-limb parts)
+    size_t sum_len;             // sums written: [0, sum_len)
     uint64_t *maxerr;           // bit pattern of the largest rounding distance (atomicMax)
+};
+
+struct FftLaunch {
+    FftLeaf one;
+    const FftLeaf *leaves;
+    int nleaves;
+    double io_bytes;
+    int logn, bits, lobits;     // digit width 16 or 8
     FftK k;
     const double2 *col_tw[MAX_PASS], *col_itw[MAX_PASS];
     const double2 *row_tw[MAX_PASS], *row_itw[MAX_PASS];
@@ -120,20 +139,71 @@ struct FftLaunch {
 hipError_t launch_fft(const FftLaunch &L, hipStream_t stream, const StageHook &hook);
 hipError_t fft_set_lds_limits();
 
+// ---- small products (pg_small.hip) --------------------------------------------
+// Schoolbook products below the transform threshold (mul_basic.cpp:131-185,
+// mul_kara.cpp; sizes <= 192 limbs, or a short side <= 24 limbs) as limb
+// sums.  Column k of x*y is sum_i x_i y_{k-i}; a workgroup computes
+// SMALL_COLS + 2 consecutive columns (3 words each) and emits the limb sums
+// of SMALL_COLS of them.
+constexpr int SMALL_THREADS = 256;
+constexpr int SMALL_COLS = SMALL_THREADS - 2;
+constexpr int SMALL_MAX_SHORT = 256;    // longest short side held in LDS
+
+struct SmallLeaf {
+    const uint64_t *a, *b;      // any order; the kernel iterates over the shorter
+    size_t an, bn;
+    uint64_t *slo, *shi;
+    size_t sum_len;
+    size_t blk0;                // first workgroup of this leaf in the launch
+};
+struct SmallLaunch {
+    SmallLeaf one;
+    const SmallLeaf *leaves;
+    const uint32_t *blk_leaf;   // leaf of every workgroup
+};
+hipError_t launch_small(const SmallLaunch &L, size_t nblocks, hipStream_t stream, const StageHook &hook,
+                        double io_bytes);
+inline size_t small_blocks(size_t an, size_t bn) { return (an + bn + SMALL_COLS - 1) / SMALL_COLS; }
+
 // ---- carry propagation (pg_carry.hip) ----------------------------------------
 constexpr int CARRY_THREADS = 256;
 constexpr int CARRY_PER_THREAD = 8;
 constexpr int CARRY_CHUNK = CARRY_THREADS * CARRY_PER_THREAD;
 
+// A block product of a blocked multiply (mul_basic.cpp:207-236): its limb
+// sums cover limbs [k*B, k*B + len) of the node's value.
+struct BlockRef {
+    const uint64_t *slo, *shi;
+    size_t len;
+};
+
+// One value to normalize.  Kinds:
+//  plain   : limb j's sum is (slo[j], shi[j]); parts == 2 when a limb is two
+//            half-limb parts (FFT tiles narrower than one limb);
+//  blocked : limb j's sum is the sum of the block products covering j (at
+//            most two: blocks are 2 lo limbs apart and at most 3 lo long).
+// The result goes to out[0, out_len) as limbs; with out_hi set it is written
+// as sums (out_hi zeroed) for a parent blocked node.
+enum { NODE_PLAIN = 0, NODE_BLOCKED = 1 };
+struct CarryNode {
+    int kind, parts;
+    const uint64_t *slo, *shi;
+    const BlockRef *blocks;
+    int nblocks;
+    size_t B;
+    size_t m;                   // limbs covered by the sums
+    uint64_t *out, *out_hi;
+    size_t out_len;
+    size_t chunk0;              // first chunk of this node in the launch
+};
+
 struct CarryLaunch {
-    const uint64_t *slo, *shi;  // limb sums, m entries (parts * m when parts == 2)
-    size_t m;                   // number of limb sums
-    int parts;                  // 1: one sum per limb; 2: two 32-bit-aligned halves per limb
-    uint64_t *out;              // normalized limbs
-    size_t out_len;             // limbs to write (>= m is allowed: zero fill)
+    CarryNode one;              // used when nodes == nullptr
+    const CarryNode *nodes;
+    const uint32_t *chunk_node; // node of every chunk
+    size_t nchunks;
     uint8_t *agg;               // per-chunk (G | P << 1)
     uint8_t *cin;               // per-chunk carry-in
-    size_t nchunks;
 };
 
 __global__ void k_carry_reduce(const CarryLaunch C);

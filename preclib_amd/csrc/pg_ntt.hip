@@ -179,12 +179,14 @@ __global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
     constexpr int DPL = 64 / mode_digit_bits(MODE);   // digits per limb
     constexpr int N1 = 1 << L1, T = 1 << LT, LPR = T / DPL;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
+    constexpr int NP = mode_primes(MODE);
     extern __shared__ uint32_t lds[];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x), op = blockIdx.y, pr = blockIdx.z;
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), op = blockIdx.y / NP, pr = blockIdx.y % NP;
     const PrimeK &c = L.pk[pr];
     const uint32_t c0 = (uint32_t)tile << LT;
-    const uint64_t *src = op ? L.b : L.a;
-    const size_t nl = op ? L.bn : L.an;
+    const uint64_t *src = op ? D.b : D.a;
+    const size_t nl = op ? D.bn : D.an;
     // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T)
     for (int e = threadIdx.x; e < N1 * LPR; e += NTT_THREADS) {
         const int j1 = e / LPR, m = e % LPR;
@@ -194,7 +196,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
     }
     __syncthreads();
     lds_forward<L1, LT, NTT_THREADS>(lds, L.col_tw[pr], c);
-    uint32_t *out = L.spec + ((size_t)(op * L.nprimes + pr) << LOGN);
+    uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
     for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
         const int rho = e >> LT, cc = e & (T - 1);
         const uint32_t ex = ((c0 + cc) * freq_of_pos<L1>(rho)) & NMASK;
@@ -207,17 +209,18 @@ __global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
 // ---- kernel 2: row pass (forward A, forward B, pointwise, inverse) -------------
 
 template <int LOGN, int MODE>
-__device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, int op, int pr, int row) {
+__device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int op, int pr,
+                                         int row) {
     constexpr int L1 = plan_l1(LOGN), L2 = plan_l2(LOGN), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN);
     if constexpr (L1 == 0) {
         // single-row plan: the row is the digit vector itself
         constexpr int DPL = 64 / mode_digit_bits(MODE);
-        const uint64_t *src = op ? L.b : L.a;
-        const size_t nl = op ? L.bn : L.an;
+        const uint64_t *src = op ? D.b : D.a;
+        const size_t nl = op ? D.bn : D.an;
         for (int m = threadIdx.x; m < N2 / DPL; m += ROW_THREADS)
             put_digits<MODE>(lds + lp(DPL * m), (size_t)m < nl ? src[m] : 0, L.crt.red[pr], L.pk[pr].p);
     } else {
-        const uint4 *in = (const uint4 *)(L.spec + ((size_t)(op * L.nprimes + pr) << LOGN) +
+        const uint4 *in = (const uint4 *)(D.spec + ((size_t)(op * mode_primes(MODE) + pr) << LOGN) +
                                           ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
             const uint4 v = in[m];
@@ -232,11 +235,12 @@ __global__ void __launch_bounds__(row_threads(LOGN)) k_row(const NttLaunch L) {
     constexpr int L2 = plan_l2(LOGN), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN);
     constexpr int E = (N2 + ROW_THREADS - 1) / ROW_THREADS;
     extern __shared__ uint32_t lds[];
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const int row = blockIdx.x, pr = blockIdx.y;
     const PrimeK &c = L.pk[pr];
     uint32_t keep[E];
 
-    load_row<LOGN, MODE>(lds, L, 0, pr, row);
+    load_row<LOGN, MODE>(lds, L, D, 0, pr, row);
     __syncthreads();
     lds_forward<L2, 0, ROW_THREADS>(lds, L.row_tw[pr], c);
 #pragma unroll
@@ -245,7 +249,7 @@ __global__ void __launch_bounds__(row_threads(LOGN)) k_row(const NttLaunch L) {
         if (i < N2) keep[k] = lds[lp(i)];
     }
     __syncthreads();
-    load_row<LOGN, MODE>(lds, L, 1, pr, row);
+    load_row<LOGN, MODE>(lds, L, D, 1, pr, row);
     __syncthreads();
     lds_forward<L2, 0, ROW_THREADS>(lds, L.row_tw[pr], c);
 #pragma unroll
@@ -255,7 +259,7 @@ __global__ void __launch_bounds__(row_threads(LOGN)) k_row(const NttLaunch L) {
     }
     __syncthreads();
     lds_inverse<L2, 0, ROW_THREADS>(lds, L.row_itw[pr], c);
-    uint4 *out = (uint4 *)(L.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
+    uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
         const uint32_t *d = lds + lp(4 * m);
         out[m] = make_uint4(d[0], d[1], d[2], d[3]);
@@ -278,12 +282,13 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t v, i
 // Inverse twiddle, 1/n and inverse column transform of prime pr's residues
 // for the tile; result left in LDS.
 template <int LOGN>
-__device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, int pr, uint32_t c0) {
+__device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int pr,
+                                           uint32_t c0) {
     constexpr int L1 = plan_l1(LOGN), L2 = plan_l2(LOGN), LT = plan_lt(LOGN);
     constexpr int T = 1 << LT, NE = (1 << L1) * T;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     const PrimeK &c = L.pk[pr];
-    const uint32_t *in = L.prod + ((size_t)pr << LOGN);
+    const uint32_t *in = D.prod + ((size_t)pr << LOGN);
     for (int e = threadIdx.x; e < NE; e += COLINV_THREADS) {
         const int rho = e >> LT, cc = e & (T - 1);
         const uint32_t ex = ((c0 + cc) * freq_of_pos<L1>(rho)) & NMASK;   // ilo/ihi hold omega^-e
@@ -302,6 +307,7 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
     constexpr int E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
     constexpr int EL = (NL + COLINV_THREADS - 1) / COLINV_THREADS;
     extern __shared__ uint32_t lds[];
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t c0 = (uint32_t)tile << LT;
     const CrtK &K = L.crt;
@@ -314,7 +320,7 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
         uint32_t res0[E];
         uint64_t coef[E];
         for (int pr = 0; pr < 2; ++pr) {
-            inv_column<LOGN>(lds, L, pr, c0);
+            inv_column<LOGN>(lds, L, D, pr, c0);
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int i = threadIdx.x + k * COLINV_THREADS;
@@ -358,7 +364,7 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
         // x = r1 + p1 t2 + p1 p2 t3 < 2^91; 2 coefficients per limb.
         uint32_t r0[E], t2[E];
         for (int pr = 0; pr < 3; ++pr) {
-            inv_column<LOGN>(lds, L, pr, c0);
+            inv_column<LOGN>(lds, L, D, pr, c0);
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int i = threadIdx.x + k * COLINV_THREADS;
@@ -438,8 +444,10 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
         if (e < NL) {
             const int rho = e / LPR, m = e % LPR;
             const size_t li = (((size_t)rho << L2) + c0 + DPL * m) / DPL;
-            L.slo[li] = slo[k];
-            L.shi[li] = shi[k];
+            if (li < D.sum_len) {
+                D.slo[li] = slo[k];
+                D.shi[li] = shi[k];
+            }
         }
     }
 }
@@ -451,24 +459,26 @@ static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const Stag
     constexpr int L1 = plan_l1(LOGN), L2 = plan_l2(LOGN), LT = plan_lt(LOGN), NP = mode_primes(MODE);
     const int ntiles = (1 << L2) >> LT;
     const size_t col_lds = lds_words((size_t)1 << (L1 + LT)) * 4, row_lds = lds_words((size_t)1 << L2) * 4;
-    const double n = (double)(1u << LOGN), io = (double)(L.an + L.bn) * 8.0;
+    const int nl = L.leaves ? L.nleaves : 1;
+    // byte-model shares use the reference's P = 2 (see bench.py)
+    const double n = (double)(1u << LOGN) * nl, io = L.io_bytes;
     if constexpr (L1 > 0) {
         h.begin(h.ctx, "ntt_col_fwd");
-        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE>), dim3(ntiles, 2, NP), dim3(NTT_THREADS), col_lds, stream, L);
+        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE>), dim3(ntiles, 2 * NP, nl), dim3(NTT_THREADS), col_lds, stream, L);
         h.end(h.ctx, "ntt_col_fwd", io + 16.0 * NP * n);
     }
     h.begin(h.ctx, "ntt_row");
-    hipLaunchKernelGGL((k_row<LOGN, MODE>), dim3(1 << L1, NP), dim3(row_threads(LOGN)), row_lds, stream, L);
+    hipLaunchKernelGGL((k_row<LOGN, MODE>), dim3(1 << L1, NP, nl), dim3(row_threads(LOGN)), row_lds, stream, L);
     h.end(h.ctx, "ntt_row", 24.0 * NP * n + (L1 ? 0.0 : io + 16.0 * NP * n));
     h.begin(h.ctx, "ntt_col_inv");
-    hipLaunchKernelGGL((k_col_inv<LOGN, MODE>), dim3(ntiles), dim3(COLINV_THREADS), col_lds, stream, L);
+    hipLaunchKernelGGL((k_col_inv<LOGN, MODE>), dim3(ntiles, 1, nl), dim3(COLINV_THREADS), col_lds, stream, L);
     h.end(h.ctx, "ntt_col_inv", 8.0 * NP * n);
     return hipGetLastError();
 }
 
 template <int LOGN, int MODE>
 static hipError_t set_limits() {
-    constexpr int lim = 160 * 1024;
+    constexpr int lim = 156 * 1024;   // dynamic LDS limit; leaves room for static LDS
     hipError_t e = hipSuccess;
     if (plan_l1(LOGN) > 0)
         e = hipFuncSetAttribute((const void *)k_col_fwd<LOGN, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);

@@ -397,12 +397,13 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
                  Scratch &s, int parts = 1) {
     Context &C = ctx();
     CarryLaunch cl{};
-    cl.parts = parts;
-    cl.slo = slo;
-    cl.shi = shi;
-    cl.m = m;
-    cl.out = out;
-    cl.out_len = out_len;
+    cl.one.kind = NODE_PLAIN;
+    cl.one.parts = parts;
+    cl.one.slo = slo;
+    cl.one.shi = shi;
+    cl.one.m = m;
+    cl.one.out = out;
+    cl.one.out_len = out_len;
     cl.nchunks = (std::max(m, out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
     cl.agg = s.get<uint8_t>(cl.nchunks);
     cl.cin = s.get<uint8_t>(cl.nchunks);
@@ -413,6 +414,76 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
     hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, C.stream, cl);
     hipLaunchKernelGGL(k_carry_apply, dim3(cl.nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
     C.stage_end("carry", e0, (double)m * 32.0 + (double)out_len * 8.0);
+    HIP_TRY(hipGetLastError());
+    return PG_OK;
+}
+
+int fill_ntt_tables(NttLaunch &L) {
+    Context &C = ctx();
+    const int logn = L.logn, NP = mode_primes(L.mode);
+    L.nprimes = NP;
+    L.crt = C.crt;
+    const int l1 = plan_l1(logn), l2 = plan_l2(logn);
+    for (int k = 0; k < NP; ++k) {
+        L.pk[k] = C.pk[k];
+        int logB = l1;
+        for (int i = 0; i < split_np(l1); ++i) {
+            const TwTables &t = C.tw(k, logB, split_lr(l1, i));
+            L.col_tw[k][i] = t.fwd; L.col_itw[k][i] = t.inv;
+            logB -= split_lr(l1, i);
+        }
+        logB = l2;
+        for (int i = 0; i < split_np(l2); ++i) {
+            const TwTables &t = C.tw(k, logB, split_lr(l2, i));
+            L.row_tw[k][i] = t.fwd; L.row_itw[k][i] = t.inv;
+            logB -= split_lr(l2, i);
+        }
+        const GlobalTw &g = C.gtw(k, logn);
+        if (!g.glo) return set_error(PG_ENOMEM, "twiddle tables");
+        L.glo[k] = g.glo; L.ghi[k] = g.ghi; L.ilo[k] = g.ilo; L.ihi[k] = g.ihi; L.lobits = g.lobits;
+    }
+    return PG_OK;
+}
+
+int fill_fft_tables(FftLaunch &L) {
+    Context &C = ctx();
+    const int logn = L.logn, l1 = fft_l1(logn), l2 = fft_l2(logn);
+    L.k = C.fk;
+    int logB = l1;
+    for (int i = 0; i < fsplit_np(l1); ++i) {
+        const FftTw &t = C.ftw(logB, fsplit_lr(l1, i));
+        L.col_tw[i] = t.fwd; L.col_itw[i] = t.inv;
+        logB -= fsplit_lr(l1, i);
+    }
+    logB = l2;
+    for (int i = 0; i < fsplit_np(l2); ++i) {
+        const FftTw &t = C.ftw(logB, fsplit_lr(l2, i));
+        L.row_tw[i] = t.fwd; L.row_itw[i] = t.inv;
+        logB -= fsplit_lr(l2, i);
+    }
+    const FftGlobal &g = C.fgtw(logn);
+    if (!g.glo) return set_error(PG_ENOMEM, "FFT twiddle tables");
+    L.glo = g.glo; L.ghi = g.ghi; L.lobits = g.lobits;
+    return PG_OK;
+}
+
+int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dnodes, const uint32_t *dchunk,
+                       size_t nchunks, uint8_t *agg, uint8_t *cin) {
+    Context &C = ctx();
+    CarryLaunch cl{};
+    cl.nodes = dnodes;
+    cl.chunk_node = dchunk;
+    cl.nchunks = nchunks;
+    cl.agg = agg;
+    cl.cin = cin;
+    double bytes = 0;
+    for (auto &n : nodes) bytes += (double)n.m * 16.0 * (n.kind == NODE_BLOCKED ? 2 : 1) + (double)n.out_len * 8.0;
+    hipEvent_t e0 = nullptr;
+    C.stage_begin(e0);
+    hipLaunchKernelGGL(k_carry_reduce, dim3(nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
+    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, C.stream, cl);
+    hipLaunchKernelGGL(k_carry_apply, dim3(nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
+    C.stage_end("carry", e0, bytes);
     HIP_TRY(hipGetLastError());
     return PG_OK;
 }
@@ -441,36 +512,20 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     const int NP = mode_primes(mode), DPL = 64 / mode_digit_bits(mode);
     Scratch s;
     NttLaunch L{};
-    L.a = ap; L.b = bp; L.an = an; L.bn = bn;
-    L.logn = logn; L.nprimes = NP; L.mode = mode;
-    L.crt = C.crt;
-    const int l1 = plan_l1(logn), l2 = plan_l2(logn);
-    for (int k = 0; k < NP; ++k) {
-        L.pk[k] = C.pk[k];
-        int logB = l1;
-        for (int i = 0; i < split_np(l1); ++i) {
-            const TwTables &t = C.tw(k, logB, split_lr(l1, i));
-            L.col_tw[k][i] = t.fwd; L.col_itw[k][i] = t.inv;
-            logB -= split_lr(l1, i);
-        }
-        logB = l2;
-        for (int i = 0; i < split_np(l2); ++i) {
-            const TwTables &t = C.tw(k, logB, split_lr(l2, i));
-            L.row_tw[k][i] = t.fwd; L.row_itw[k][i] = t.inv;
-            logB -= split_lr(l2, i);
-        }
-        const GlobalTw &g = C.gtw(k, logn);
-        if (!g.glo) return set_error(PG_ENOMEM, "twiddle tables");
-        L.glo[k] = g.glo; L.ghi[k] = g.ghi; L.ilo[k] = g.ilo; L.ihi[k] = g.ihi; L.lobits = g.lobits;
-    }
-    if (l1) L.spec = s.get<uint32_t>(2 * NP * n);
-    L.prod = s.get<uint32_t>(NP * n);
+    L.one.a = ap; L.one.b = bp; L.one.an = an; L.one.bn = bn;
+    L.io_bytes = (double)(an + bn) * 16.0;
+    L.logn = logn; L.mode = mode;
+    { int st = fill_ntt_tables(L); if (st) return st; }
+    const int l1 = plan_l1(logn);
+    if (l1) L.one.spec = s.get<uint32_t>(2 * NP * n);
+    L.one.prod = s.get<uint32_t>(NP * n);
     const size_t m = n / DPL;
-    L.slo = s.get<uint64_t>(m);
-    L.shi = s.get<uint64_t>(m);
-    if ((l1 && !L.spec) || !L.prod || !L.slo || !L.shi) return set_error(PG_ENOMEM, "NTT scratch");
+    L.one.slo = s.get<uint64_t>(m);
+    L.one.shi = s.get<uint64_t>(m);
+    L.one.sum_len = m;
+    if ((l1 && !L.one.spec) || !L.one.prod || !L.one.slo || !L.one.shi) return set_error(PG_ENOMEM, "NTT scratch");
     HIP_TRY(launch_ntt(L, C.stream, C.hook()));
-    return launch_carry(L.slo, L.shi, m, rp, an + bn, s);
+    return launch_carry(L.one.slo, L.one.shi, m, rp, an + bn, s);
 }
 
 }  // namespace pg
@@ -501,32 +556,19 @@ int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     const int parts = dpl / seg;
     Scratch s;
     FftLaunch L{};
-    L.a = ap; L.b = bp; L.an = an; L.bn = bn;
+    L.one.a = ap; L.one.b = bp; L.one.an = an; L.one.bn = bn;
+    L.io_bytes = (double)(an + bn) * 16.0;
     L.logn = logn; L.bits = bits;
-    L.k = C.fk;
-    L.maxerr = C.maxerr;
-    int logB = l1;
-    for (int i = 0; i < fsplit_np(l1); ++i) {
-        const FftTw &t = C.ftw(logB, fsplit_lr(l1, i));
-        L.col_tw[i] = t.fwd; L.col_itw[i] = t.inv;
-        logB -= fsplit_lr(l1, i);
-    }
-    logB = l2;
-    for (int i = 0; i < fsplit_np(l2); ++i) {
-        const FftTw &t = C.ftw(logB, fsplit_lr(l2, i));
-        L.row_tw[i] = t.fwd; L.row_itw[i] = t.inv;
-        logB -= fsplit_lr(l2, i);
-    }
-    const FftGlobal &g = C.fgtw(logn);
-    if (!g.glo) return set_error(PG_ENOMEM, "FFT twiddle tables");
-    L.glo = g.glo; L.ghi = g.ghi; L.lobits = g.lobits;
-    L.spec = s.get<double2>(n);
+    L.one.maxerr = C.maxerr;
+    { int st = fill_fft_tables(L); if (st) return st; }
+    L.one.spec = s.get<double2>(n);
     const size_t nsum = n / seg;
-    L.slo = s.get<uint64_t>(nsum);
-    L.shi = s.get<uint64_t>(nsum);
-    if (!L.spec || !L.slo || !L.shi) return set_error(PG_ENOMEM, "FFT scratch");
+    L.one.slo = s.get<uint64_t>(nsum);
+    L.one.shi = s.get<uint64_t>(nsum);
+    L.one.sum_len = nsum;
+    if (!L.one.spec || !L.one.slo || !L.one.shi) return set_error(PG_ENOMEM, "FFT scratch");
     HIP_TRY(launch_fft(L, C.stream, C.hook()));
-    int st = launch_carry(L.slo, L.shi, n / dpl, rp, an + bn, s, parts);
+    int st = launch_carry(L.one.slo, L.one.shi, n / dpl, rp, an + bn, s, parts);
     if (st) return st;
     if (max_round) {
         uint64_t bitsv = 0;

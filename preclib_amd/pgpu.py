@@ -20,7 +20,15 @@ SZ = ctypes.c_size_t
 # Every symbol include/pgpu.h declares.
 EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", "pg_buf_free",
            "pg_buf_capacity", "pg_buf_upload", "pg_buf_download", "pg_buf_ptr", "pg_mul_dev",
+           "pg_mul_batch", "pg_batch_plan", "pg_batch_run", "pg_batch_errors", "pg_batch_free",
            "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read"]
+
+
+class Job(ctypes.Structure):
+    """pg_job (include/pgpu.h)."""
+    _fields_ = [("a", ctypes.c_void_p), ("an", SZ), ("b", ctypes.c_void_p), ("bn", SZ),
+                ("r", ctypes.c_void_p), ("r_cap", SZ), ("r_len", SZ), ("path", ctypes.c_int),
+                ("status", ctypes.c_int), ("fft_max_round", ctypes.c_double)]
 
 
 class PgError(RuntimeError):
@@ -60,6 +68,11 @@ def load(path=None):
     L.pg_buf_download.argtypes = [ctypes.c_void_p, P64, SZ]
     L.pg_mul_dev.argtypes = [ctypes.c_void_p, ctypes.POINTER(SZ), ctypes.c_void_p, SZ, ctypes.c_void_p,
                              SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.pg_mul_batch.argtypes = [ctypes.POINTER(Job), SZ]
+    L.pg_batch_plan.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(Job), SZ]
+    L.pg_batch_run.argtypes = [ctypes.c_void_p]
+    L.pg_batch_errors.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), SZ]
+    L.pg_batch_free.argtypes = [ctypes.c_void_p]
     L.pg_timing_enable.argtypes = [ctypes.c_int]
     L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
     _lib = L
@@ -174,3 +187,58 @@ def timing_read():
     arr = (StageTime * max(n, 1))()
     n = L.pg_timing_read(arr, n)
     return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms, arr[i].bytes) for i in range(n)}
+
+
+def mul_batch(pairs, path=PATH_AUTO, want_round=False):
+    """Products of many (a, b) pairs in one submission (host buffers)."""
+    if not _inited:
+        init()
+    keep, jobs = [], (Job * max(len(pairs), 1))()
+    outs = []
+    for i, (a, b) in enumerate(pairs):
+        a, b = _trimmed(a), _trimmed(b)
+        r = np.zeros(len(a) + len(b), np.uint64)
+        keep += [a, b]
+        outs.append(r)
+        jobs[i] = Job(a.ctypes.data, len(a), b.ctypes.data, len(b), r.ctypes.data, len(r), 0, path, 0, 0.0)
+    _check(_lib.pg_mul_batch(jobs, len(pairs)))
+    res = [outs[i][: jobs[i].r_len] for i in range(len(pairs))]
+    if want_round:
+        return res, [jobs[i].fft_max_round for i in range(len(pairs))]
+    return res
+
+
+class DevBatch:
+    """A planned device-resident batch (pg_batch): products of DevBuf pairs."""
+
+    def __init__(self, triples, path=PATH_AUTO):
+        """triples: list of (r: DevBuf, a: DevBuf, an, b: DevBuf, bn)."""
+        if not _inited:
+            init()
+        self.n = len(triples)
+        jobs = (Job * max(self.n, 1))()
+        for i, (r, a, an, b, bn) in enumerate(triples):
+            jobs[i] = Job(_lib.pg_buf_ptr(a.h), an, _lib.pg_buf_ptr(b.h), bn, _lib.pg_buf_ptr(r.h), r.cap, 0,
+                          path, 0, 0.0)
+        h = ctypes.c_void_p()
+        _check(_lib.pg_batch_plan(ctypes.byref(h), jobs, self.n))
+        self.h = h
+
+    def run(self):
+        _check(_lib.pg_batch_run(self.h))
+
+    def errors(self):
+        e = (ctypes.c_double * max(self.n, 1))()
+        _check(_lib.pg_batch_errors(self.h, e, self.n))
+        return [e[i] for i in range(self.n)]
+
+    def free(self):
+        if self.h:
+            _lib.pg_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,101 @@
+"""GPU parity of the dispatcher paths that need more than one transform:
+small products, blocked (unbalanced) products, and batches of uneven
+products in one submission (plain host API and planned device batches
+replayed from a hipGraph)."""
+import hashlib
+import json
+
+import numpy as np
+import pytest
+
+from preclib_amd import workloads as W
+from tests.oracle_py import digest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pg():
+    from preclib_amd import pgpu
+    pgpu.init(0)
+    return pgpu
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 500), (5, 300), (24, 24), (25, 192), (192, 192), (30, 100),
+                                   (193, 1000), (500, 7000), (25, 50000), (300, 100), (1000, 2100),
+                                   (2049, 9000), (7000, 20000), (6145, 30000), (200, 100000)])
+@pytest.mark.parametrize("dist", ["uniform", "ones"])
+def test_auto_vs_oracle(pg, oracle, shape, dist):
+    """mul_into's full dispatch (schoolbook, blocked, NTT, FFT) on the device."""
+    an, bn = shape
+    a, b = W.pair(an, bn, dist, k=an + 7)
+    print("case", an, bn, dist, flush=True)
+    got = pg.mul(a, b)
+    assert np.array_equal(got, oracle.mul_into(a, b))
+
+
+def test_reference_all_mul_eq_grid(pg, golden):
+    """test_prec.cpp:429-478: sizes 32..2048 x ratios 1..3 (32-bit LCG limbs)."""
+    for e in golden["all_mul_eq"]:
+        a = W.reference_pattern32(e["an"], e["seed"])
+        b = W.reference_pattern32(e["bn"], e["seed"] + 1009)
+        assert digest(pg.mul(a, b)) == e["sha256"], e
+
+
+def test_bench_gmp_rows(pg, golden):
+    """bench_gmp.cpp:104-129 rows (64-bit LCG limbs) through mul_into."""
+    for e in golden["bench_gmp"]:
+        a = W.reference_pattern(e["an"], 11 + e["p"])
+        b = W.reference_pattern(e["bn"], 19 + e["p"])
+        assert digest(pg.mul(a, b)) == e["sha256"], e
+
+
+def test_batch_mixed_equals_alone(pg, oracle):
+    """Every product of a mixed batch equals the same product alone."""
+    shapes = [(30, 100), (193, 193), (500, 700), (1000, 2500), (7000, 7000), (100, 50), (600, 6000),
+              (8000, 3000), (20000, 9000), (5, 70000), (65536, 65536), (300, 300)]
+    pairs = [W.pair(an, bn, "uniform", k=i) for i, (an, bn) in enumerate(shapes)]
+    res, errs = pg.mul_batch(pairs, want_round=True)
+    for (a, b), r, e in zip(pairs, res, errs):
+        alone = pg.mul(a, b)
+        assert np.array_equal(r, alone)
+        assert np.array_equal(r, oracle.mul_into(a, b))
+        assert 0.0 <= e < 0.125
+
+
+def test_uneven_batch_4096_golden(pg, golden):
+    """The uneven batch: 4096 products, sizes log-uniform in [193, 65536],
+    30% unbalanced up to 4:1; digest of the per-product digests."""
+    shapes = W.batch_shapes()
+    assert hashlib.sha256(json.dumps(shapes).encode()).hexdigest() == golden["batch"]["shapes_sha256"]
+    pairs = [W.pair(an, bn, "uniform", k) for k, (an, bn) in enumerate(shapes)]
+    res = pg.mul_batch(pairs)
+    per = [digest(r) for r in res]
+    assert per[:64] == golden["batch"]["first"]
+    assert hashlib.sha256("".join(per).encode()).hexdigest() == golden["batch"]["digest_of_digests"]
+
+
+def test_device_batch_graph_replays(pg, oracle):
+    """A planned device batch is captured into a hipGraph on its first run and
+    replayed; every replay, with eager work and syncs in between, equals the
+    eager result."""
+    shapes = [(300, 700), (5000, 5000), (9000, 2500), (20, 4000), (70000, 70000)]
+    bufs, trip, want = [], [], []
+    for i, (an, bn) in enumerate(shapes):
+        a, b = W.pair(an, bn, "uniform", k=100 + i)
+        da, db, dr = pg.DevBuf.of(a), pg.DevBuf.of(b), pg.DevBuf(an + bn)
+        bufs += [da, db, dr]
+        trip.append((dr, da, an, db, bn))
+        want.append(oracle.mul_into(a, b))
+    batch = pg.DevBatch(trip)
+    for rep in range(3):
+        for (dr, _, an, _, bn) in trip:            # clobber the outputs between replays
+            dr.upload(np.full(an + bn, 0xAB, np.uint64))
+        batch.run()
+        pg.sync()
+        for (dr, _, an, _, bn), w in zip(trip, want):
+            got = dr.download(an + bn)
+            assert np.array_equal(got[: len(w)], w) and not got[len(w):].any(), rep
+        x, y = W.pair(3000, 3000, "uniform", k=rep)   # eager work between replays
+        assert np.array_equal(pg.mul(x, y), oracle.mul_into(x, y))
+    batch.free()
