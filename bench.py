@@ -95,6 +95,127 @@ def cpu_baseline(a, b, budget_s=25.0):
                       % (len(times), len(a), len(b), med)}
 
 
+def ref_leaf_bytes(an, bn, sqr=False):
+    """Byte model of one product under the reference's own decomposition
+    (mul_basic.cpp:238-276): NTT leaf 16(Na+Nb) + 48*2*n (base-2^16 n), FFT
+    leaf 16(Na+Nb) + 128 n, blocked = sum of its blocks + 16 (block + lo) per
+    block accumulate, schoolbook leaf 16(Na+Nb)."""
+    lo, hi = min(an, bn), max(an, bn)
+    if lo <= 24 or hi <= 192:
+        return 16.0 * (an + bn)
+    if hi > 2 * lo and not sqr:
+        tot, B = 0.0, 2 * lo
+        for off in range(0, hi, B):
+            pn = min(B, hi - off)
+            tot += ref_leaf_bytes(pn, lo) + 16.0 * (pn + lo)
+        return tot
+    def np2(x):
+        n = 1
+        while n < x:
+            n <<= 1
+        return n
+    if hi > 6144 and not sqr:
+        return 16.0 * (an + bn) + 96.0 * np2(4 * (an + bn))
+    dpl = 4 if hi <= 1024 else 8
+    return 16.0 * (an + bn) + 128.0 * np2(dpl * (an + bn))
+
+
+def bench_batch(args, pgpu, workloads):
+    """Uneven batch: plain per-product launches vs one planned, graph-replayed
+    batch; both device-resident."""
+    shapes = workloads.batch_shapes()
+    pairs = [workloads.pair(an, bn, "uniform", k) for k, (an, bn) in enumerate(shapes)]
+    limbs = sum(an + bn for an, bn in shapes)
+    dev = []
+    for (a, b) in pairs:
+        dev.append((pgpu.DevBuf(len(a) + len(b)), pgpu.DevBuf.of(a), len(a), pgpu.DevBuf.of(b), len(b)))
+    # plain: one product at a time through the dispatcher
+    for t in dev[:64]:
+        pgpu.mul_dev(t[0], t[1], t[2], t[3], t[4], want_len=False)
+    pgpu.sync()
+    t0 = time.perf_counter()
+    for t in dev:
+        pgpu.mul_dev(t[0], t[1], t[2], t[3], t[4], want_len=False)
+    pgpu.sync()
+    plain_s = time.perf_counter() - t0
+    # batched: plan once, replay the captured graph
+    batch = pgpu.DevBatch(dev)
+    for _ in range(args.warmup):
+        batch.run()
+    pgpu.sync()
+    per = []
+    for (r, _, an, _, bn) in dev:
+        x = r.download(an + bn)
+        n = len(x)
+        while n and x[n - 1] == 0:
+            n -= 1
+        per.append(hashlib.sha256(x[:n].astype("<u8").tobytes()).hexdigest())
+    check = None
+    gold = os.path.join(ROOT, "tests", "golden", "products.json")
+    if os.path.exists(gold):
+        check = hashlib.sha256("".join(per).encode()).hexdigest() == json.load(open(gold))["batch"]["digest_of_digests"]
+    if check is False:
+        print("bench.py: batch differs from the golden digests", file=sys.stderr)
+        sys.exit(1)
+    pgpu.timing(True)
+    pgpu.timing_reset()
+    pgpu.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    pgpu.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    stages = pgpu.timing_read()
+    pgpu.timing(False)
+    # untimed-events rate (graph replay; events disable the graph path)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    pgpu.sync()
+    dt_graph = (time.perf_counter() - t0) / args.steps
+    model = sum(ref_leaf_bytes(an, bn) for an, bn in shapes)
+    best = min(dt, dt_graph)
+    line = {
+        "metric": "product limbs/s, 4096 uneven products in one submission (batched vs plain)",
+        "value": limbs / best, "unit": "product limbs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32 modular (NTT) / f64 (FFT), u64 limbs",
+        "data": "synthetic (splitmix64 uniform limbs)",
+        "config": {"workload": "4096 products, Nhi log-uniform [193, 65536], 30% unbalanced up to 4:1",
+                   "limbs": limbs, "bit_exact_vs_golden": check},
+        "plain": {"value": limbs / plain_s, "ms": plain_s * 1e3},
+        "batched_over_plain": plain_s / best,
+        "graph_replay_ms": dt_graph * 1e3, "eager_with_events_ms": dt * 1e3,
+        "product_roofline": {"bytes": model, "achieved_GBps": model / best / 1e9, "frac": model / best / HBM_PEAK},
+        "stages_ms_per_step": {k: v[1] / args.steps for k, v in stages.items()},
+    }
+    if not args.no_cpu:
+        import ctypes
+        ref = os.path.join(ROOT, "oracle", "_ref", "libpreclib_ref.so")
+        if os.path.exists(ref):
+            L = ctypes.CDLL(ref)
+            f = L.ref_mul_into
+            P64 = ctypes.POINTER(ctypes.c_uint64)
+            f.restype = ctypes.c_size_t
+            f.argtypes = [P64, ctypes.c_size_t, P64, ctypes.c_size_t, P64, ctypes.c_size_t, ctypes.c_int]
+            try:
+                os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+            except Exception:
+                pass
+            done, t0 = 0, time.perf_counter()
+            for (a, b) in pairs:
+                r = np.zeros(len(a) + len(b), np.uint64)
+                f(r.ctypes.data_as(P64), len(r), a.ctypes.data_as(P64), len(a), b.ctypes.data_as(P64), len(b), 0)
+                done += len(a) + len(b)
+                if time.perf_counter() - t0 > 20.0:
+                    break
+            el = time.perf_counter() - t0
+            line["cpu_baseline"] = {"value": done / el, "unit": "product limbs/s", "cores": 1, "kind": "reference",
+                                    "sample": "first %d limbs of the batch in order (%.1f s), one host core" % (done, el)}
+    print(json.dumps(line), flush=True)
+    batch.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +224,8 @@ def main():
     ap.add_argument("--logn", type=int, default=22, help="operand size 2^logn limbs")
     ap.add_argument("--dist", default="uniform")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="single", choices=["single", "batch"],
+                    help="single: one 2^22 x 2^22 product (default); batch: the 4096-product uneven batch")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -120,6 +243,8 @@ def main():
         torch.cuda.synchronize()
         del x, y
     pgpu.init(0)
+    if args.workload == "batch":
+        return bench_batch(args, pgpu, workloads)
     N = 1 << args.logn
     a, b = workloads.pair(N, N, args.dist)
     n = transform_len(a, b)
