@@ -25,7 +25,8 @@ enum {
     PG_ENOMEM = -2,    /* device memory budget or hipMalloc exhausted        */
     PG_EDEVICE = -3,   /* HIP runtime error                                  */
     PG_ERANGE = -4,    /* operand outside the path's supported size range    */
-    PG_ENOINIT = -5    /* pg_init not called                                 */
+    PG_ENOINIT = -5,   /* pg_init not called                                 */
+    PG_EINTERNAL = -6  /* a checked invariant failed (result not produced)   */
 };
 
 /* Product paths.  PG_PATH_AUTO applies preclib's dispatcher rules for sizes
@@ -117,6 +118,24 @@ int pg_batch_plan(pg_batch **out, const pg_job *jobs, size_t n);
 int pg_batch_run(pg_batch *b);
 int pg_batch_errors(pg_batch *b, double *fft_max_round, size_t n);   /* per job; waits */
 int pg_batch_free(pg_batch *b);
+
+/* Newton division, div_mulinv / mod_mulinv (src/div_newton.cpp:89-139,
+ * declared at prec.hpp:151-152): q = floor(a / b), r = a mod b.  b == 0 or
+ * a == 0 gives q = r = 0 (empty), as div_mulinv_impl does (div_newton.cpp:
+ * 90-93).  Operands need not be trimmed.  q or r may be NULL when not wanted;
+ * q needs room for an - bn + 1 limbs, r for bn.  Outputs are trimmed and the
+ * lengths written to q_len / r_len.  Every product runs on the GPU; the
+ * reciprocal follows newton64_reciprocal_approx (div_newton.cpp:36-68) and
+ * the quotient correction div_newton.cpp:114-125.  Waits for the device. */
+int pg_div(uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *r, size_t r_cap, size_t *r_len,
+           const uint64_t *a, size_t an, const uint64_t *b, size_t bn);
+int pg_div_dev(pg_buf *q, size_t q_cap, size_t *q_len, pg_buf *r, size_t r_cap, size_t *r_len,
+               const pg_buf *a, size_t an, const pg_buf *b, size_t bn);
+
+/* precn_reciprocal_newton (src/div_newton.cpp:70-87, prec.hpp:150): the
+ * exact floor(2^bits / b); 0 (empty) when b == 0 or bits < bit_length(b). */
+int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t bn, size_t bits);
+int pg_recip_dev(pg_buf *x, size_t x_cap, size_t *x_len, const pg_buf *b, size_t bn, size_t bits);
 
 /* Waits for all queued device work. */
 int pg_sync(void);

@@ -70,6 +70,17 @@ size_t or_mul_into(uint64_t *r, size_t rcap, const uint64_t *a, size_t an,
                    const uint64_t *b, size_t bn, int sqr);
 size_t or_sqr(uint64_t *r, size_t rcap, const uint64_t *a, size_t an);
 
+/* ---- division (src/div_basic.cpp, src/div_newton.cpp, src/sqrt.cpp) --- */
+/* floor(a / b) and a mod b, both trimmed (lengths in *qlen / *rlen; r may be
+ * NULL).  b == 0 or a == 0 gives empty results (div_newton.cpp:90-93).
+ * Returns 0, or (size_t)-1 if a capacity is too small. */
+size_t or_divmod(uint64_t *q, size_t qcap, size_t *qlen, uint64_t *r, size_t rcap, size_t *rlen,
+                 const uint64_t *a, size_t an, const uint64_t *b, size_t bn);
+/* floor(2^bits / b); 0 when b == 0 or bits < bit_length(b) (div_newton.cpp:70-87). */
+size_t or_recip(uint64_t *x, size_t xcap, const uint64_t *b, size_t bn, size_t bits);
+/* floor(sqrt(a)) (sqrt.cpp:71-86). */
+size_t or_sqrt(uint64_t *x, size_t xcap, const uint64_t *a, size_t an);
+
 /* ---- synthetic operands ------------------------------------------------ */
 /* splitmix64 stream (Vigna's published generator). dist: 0 uniform, 1 all
  * ones, 2 sparse (1% nonzero).  The top limb is always nonzero. */

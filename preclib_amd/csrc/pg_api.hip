@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cstring>
 #include "pg_batch.h"
+#include "pg_dint.h"
 #include "pg_runtime.h"
 #include "pgpu.h"
 
@@ -239,6 +240,87 @@ int pg_mul_batch(pg_job *jobs, size_t n) {
     return st;
 }
 
+// ---- Newton division (div_newton.cpp) -------------------------------------------
+
+static size_t host_trim(const uint64_t *a, size_t n) {
+    while (n && !a[n - 1]) --n;
+    return n;
+}
+
+static int finish_out(const DBuf &src, size_t len, uint64_t *dst, size_t cap, size_t *out_len, bool device) {
+    if (out_len) *out_len = len;
+    if (!dst) return PG_OK;
+    if (len > cap) return set_error(PG_EINVAL, "output capacity too small");
+    Context &C = ctx();
+    if (len)
+        HIP_TRY(hipMemcpyAsync(dst, src.d, len * 8, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                               C.stream));
+    return C.sync();
+}
+
+int pg_div_dev(pg_buf *q, size_t q_cap, size_t *q_len, pg_buf *r, size_t r_cap, size_t *r_len, const pg_buf *a,
+               size_t an, const pg_buf *b, size_t bn) {
+    LOCK;
+    NEED_INIT;
+    if (!a || !b || an > a->cap || bn > b->cap || (q && q_cap > q->cap) || (r && r_cap > r->cap))
+        return set_error(PG_EINVAL, "pg_div_dev: bad argument");
+    size_t at = 0, bt = 0;
+    int st = top_len(a->d, an, &at);
+    if (!st) st = top_len(b->d, bn, &bt);
+    if (st) return st;
+    DBuf dq, dr;
+    size_t ql = 0, rl = 0;
+    if ((st = div_mulinv(dq, ql, r ? &dr : nullptr, &rl, a->d, at, b->d, bt))) return st;
+    if ((st = finish_out(dq, ql, q ? q->d : nullptr, q_cap, q_len, true))) return st;
+    return finish_out(dr, rl, r ? r->d : nullptr, r_cap, r_len, true);
+}
+
+int pg_div(uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a,
+           size_t an, const uint64_t *b, size_t bn) {
+    LOCK;
+    NEED_INIT;
+    if ((an && !a) || (bn && !b)) return set_error(PG_EINVAL, "pg_div: bad argument");
+    an = host_trim(a, an);
+    bn = host_trim(b, bn);
+    Context &C = ctx();
+    DBuf da(an), db(bn), dq, dr;
+    if (!da.d || !db.d) return set_error(PG_ENOMEM, "pg_div: operands");
+    if (an) HIP_TRY(hipMemcpyAsync(da.d, a, an * 8, hipMemcpyHostToDevice, C.stream));
+    if (bn) HIP_TRY(hipMemcpyAsync(db.d, b, bn * 8, hipMemcpyHostToDevice, C.stream));
+    size_t ql = 0, rl = 0;
+    int st = div_mulinv(dq, ql, r ? &dr : nullptr, &rl, da.d, an, db.d, bn);
+    if (st) return st;
+    if ((st = finish_out(dq, ql, q, q_cap, q_len, false))) return st;
+    return finish_out(dr, rl, r, r_cap, r_len, false);
+}
+
+int pg_recip_dev(pg_buf *x, size_t x_cap, size_t *x_len, const pg_buf *b, size_t bn, size_t bits) {
+    LOCK;
+    NEED_INIT;
+    if (!x || !b || bn > b->cap || x_cap > x->cap) return set_error(PG_EINVAL, "pg_recip_dev: bad argument");
+    size_t bt = 0, xl = 0;
+    int st = top_len(b->d, bn, &bt);
+    if (st) return st;
+    DBuf dx;
+    if ((st = recip_exact(dx, xl, b->d, bt, bits))) return st;
+    return finish_out(dx, xl, x->d, x_cap, x_len, true);
+}
+
+int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t bn, size_t bits) {
+    LOCK;
+    NEED_INIT;
+    if (bn && !b) return set_error(PG_EINVAL, "pg_recip: bad argument");
+    bn = host_trim(b, bn);
+    Context &C = ctx();
+    DBuf db(bn), dx;
+    if (!db.d) return set_error(PG_ENOMEM, "pg_recip: operand");
+    if (bn) HIP_TRY(hipMemcpyAsync(db.d, b, bn * 8, hipMemcpyHostToDevice, C.stream));
+    size_t xl = 0;
+    int st = recip_exact(dx, xl, db.d, bn, bits);
+    if (st) return st;
+    return finish_out(dx, xl, x, x_cap, x_len, false);
+}
+
 int pg_timing_enable(int on) {
     LOCK;
     ctx().timing = on != 0;
@@ -308,6 +390,10 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
         if (st != PG_ERANGE) return st;
         path = PG_PATH_FFT;   // beyond the NTT caps the reference multiplies with mul_fft
     }
-    return fft_mul_device(rp, ap, an, atop, bp, bn, btop, fft_max_round);
+    const int st = fft_mul_device(rp, ap, an, atop, bp, bn, btop, fft_max_round);
+    // precn_sqr sends every square to mul_fft (mul_basic.cpp:203); past the
+    // FFT's 2^24-point table the same exact product comes from the NTT.
+    if (st == PG_ERANGE && sqr) return ntt_mul_device(rp, ap, an, atop, bp, bn, btop, MODE_WIDE);
+    return st;
 }
 }  // namespace pg

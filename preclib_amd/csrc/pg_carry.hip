@@ -14,6 +14,9 @@
 // walks CARRY_ITERS such groups (coalesced loads, the wave carry chained in
 // registers); chunks of CARRY_CHUNK limbs are joined by a per-chunk
 // aggregate and a one-workgroup scan.  Three launches: reduce, scan, apply.
+// Linear nodes reuse the same scan for device additions, subtractions,
+// shifts and small-scalar multiples (the add/sub/shift/mul_u32 steps that
+// div_newton.cpp, sqrt.cpp and pi_chudnovsky.cpp run between products).
 #include "pg_kernels.h"
 
 namespace pg {
@@ -29,8 +32,44 @@ __device__ __forceinline__ const CarryNode &node_of(const CarryLaunch &C, size_t
     return C.nodes ? C.nodes[C.chunk_node[chunk]] : C.one;
 }
 
+// Limb j of v (n limbs, zero beyond) shifted right by sh bits (left for sh < 0).
+__device__ __forceinline__ uint64_t shifted_limb(const uint64_t *v, size_t n, int64_t sh, size_t j) {
+    if (!v) return 0;
+    if (sh >= 0) {
+        const size_t i = j + ((size_t)sh >> 6);
+        const int r = (int)(sh & 63);
+        uint64_t w = i < n ? v[i] : 0;
+        if (r) w = (w >> r) | ((i + 1 < n ? v[i + 1] : 0) << (64 - r));
+        return w;
+    }
+    const size_t q = (size_t)(-sh) >> 6;
+    const int r = (int)((-sh) & 63);
+    uint64_t w = (j >= q && j - q < n) ? v[j - q] : 0;
+    if (r) w = (w << r) | ((j >= q + 1 && j - q - 1 < n) ? v[j - q - 1] : 0) >> (64 - r);
+    return w;
+}
+
+__device__ __forceinline__ bool lin_cond(const CarryNode &N) { return !N.cond || *N.cond == N.cond_eq; }
+
 // Limb sum j of a node as (lo, hi).
 __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t &lo, uint64_t &hi) {
+    if (N.kind == NODE_LIN) {
+        const uint64_t xv = shifted_limb(N.x, N.xn, N.xsh, j);
+        lo = xv * N.xmul;
+        hi = __umul64hi(xv, N.xmul);
+        if (N.pow2 >= 0 && j == ((size_t)N.pow2 >> 6)) {
+            const uint64_t b = 1ull << (N.pow2 & 63);
+            lo += b;
+            hi += lo < b;
+        }
+        if (lin_cond(N)) {
+            uint64_t yv = shifted_limb(N.y, N.yn, N.ysh, j);
+            if (N.yneg) yv = ~yv;
+            lo += yv;
+            hi += lo < yv;
+        }
+        return;
+    }
     if (N.kind == NODE_PLAIN) {
         if (N.parts == 1) {
             lo = N.slo[j];
@@ -121,6 +160,11 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_reduce(const CarryLaunc
 __device__ __forceinline__ bool node_start(const CarryLaunch &C, size_t i) {
     return C.nodes ? C.nodes[C.chunk_node[i]].chunk0 == i : i == 0;
 }
+// Carry into the lowest limb of the node that starts at chunk i.
+__device__ __forceinline__ uint32_t node_cin(const CarryLaunch &C, size_t i) {
+    const CarryNode &N = node_of(C, i);
+    return (N.kind == NODE_LIN && N.cin && lin_cond(N)) ? 1u : 0u;
+}
 
 __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     __shared__ Agg waves[16];
@@ -130,7 +174,7 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     Agg a{0, 1};
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
         const uint32_t v = C.agg[i];
-        if (node_start(C, i)) a = Agg{0, 0};
+        if (node_start(C, i)) a = Agg{node_cin(C, i), 0};
         a = compose(a, Agg{v & 1, v >> 1});
     }
     // thread-level (G, P) pairs -> carries via the wave sum trick
@@ -148,7 +192,7 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     const uint64_t sum = G.X + G.Y + wave_in[threadIdx.x >> 6];
     uint32_t x = (uint32_t)(((sum ^ G.X ^ G.Y) >> (threadIdx.x & 63)) & 1);
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
-        if (node_start(C, i)) x = 0;
+        if (node_start(C, i)) x = node_cin(C, i);
         C.cin[i] = (uint8_t)x;
         const uint32_t v = C.agg[i];
         x = (v & 1) | ((v >> 1) & x);
@@ -191,6 +235,7 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_apply(const CarryLaunch
             N.out[j] = t[it] + x;
             if (N.out_hi) N.out_hi[j] = 0;
         }
+        if (j == N.m && N.flag) *N.flag = t[it] + x;
         cin = group_out(G[it], cin);
     }
 }

@@ -184,7 +184,15 @@ struct BlockRef {
 //            most two: blocks are 2 lo limbs apart and at most 3 lo long).
 // The result goes to out[0, out_len) as limbs; with out_hi set it is written
 // as sums (out_hi zeroed) for a parent blocked node.
-enum { NODE_PLAIN = 0, NODE_BLOCKED = 1 };
+//  linear  : value = xmul * X' + 2^pow2 + (yneg ? ~Y' : Y') + cin over m
+//            limbs, where X' / Y' are x / y shifted by xsh / ysh bits (> 0
+//            right, < 0 left), limbs outside [0, xn) / [0, yn) read as 0 and
+//            ~Y' is taken over the m limbs.  With yneg and cin = 1 this is
+//            X' - Y' in two's complement; limb m of the exact sum (1 when the
+//            difference is >= 0, the carry-out for an addition) goes to *flag.
+//            When cond is set, the Y term and cin apply only if *cond ==
+//            cond_eq (otherwise the value is xmul * X' + 2^pow2).
+enum { NODE_PLAIN = 0, NODE_BLOCKED = 1, NODE_LIN = 2 };
 struct CarryNode {
     int kind, parts;
     const uint64_t *slo, *shi;
@@ -195,6 +203,15 @@ struct CarryNode {
     uint64_t *out, *out_hi;
     size_t out_len;
     size_t chunk0;              // first chunk of this node in the launch
+    // NODE_LIN
+    const uint64_t *x, *y;
+    size_t xn, yn;
+    int64_t xsh, ysh, pow2;     // pow2 < 0: no power-of-two term
+    uint64_t xmul;              // < 2^59
+    int yneg, cin;
+    const uint64_t *cond;
+    uint64_t cond_eq;
+    uint64_t *flag;
 };
 
 struct CarryLaunch {

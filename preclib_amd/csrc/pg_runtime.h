@@ -86,6 +86,14 @@ public:
     const FftGlobal &fgtw(int logn);
     FftK fk{};
     uint64_t *maxerr = nullptr;   // device word for the FFT rounding distance
+    static constexpr size_t NWORDS = 4096;
+    uint64_t *words = nullptr;    // device flags / small results of linear steps
+
+    // Host descriptor images handed to hipMemcpyAsync stay alive until the
+    // next sync() of the stream.
+    std::vector<std::vector<char>> held;
+    void hold(std::vector<char> &&img) { held.push_back(std::move(img)); }
+    int sync();
 
     // timing
     bool timing = false;
@@ -115,6 +123,13 @@ private:
 Context &ctx();
 int set_error(int code, const std::string &msg);
 
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_error(PG_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
 // Product entry points on device pointers (stream-ordered on ctx().stream).
 // rp must hold an + bn limbs; it may alias ap or bp.  Returns a status code.
 int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
@@ -123,5 +138,9 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
 // rounding distance of this product (waits for the device).
 int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
                    const uint64_t *bp, size_t bn, uint64_t btop, double *max_round);
+// mul_into's routing (src/mul_basic.cpp:238-276) on device pointers; atop /
+// btop are the top limbs or ~0 when unknown (full top limbs assumed).
+int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,
+               size_t bn, uint64_t btop, int path, double *fft_max_round);
 
 }  // namespace pg

@@ -15,12 +15,6 @@ int set_error(int code, const std::string &msg) {
 }
 const char *last_error() { return g_err.c_str(); }
 
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return set_error(PG_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
-    } while (0)
 
 uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {
     uint64_t r = 1, x = b % p;
@@ -140,6 +134,7 @@ int Context::init(int dev, size_t budget) {
     HIP_TRY(ntt_set_lds_limits());
     HIP_TRY(fft_set_lds_limits());
     HIP_TRY(hipMalloc(&maxerr, 256));
+    HIP_TRY(hipMalloc(&words, NWORDS * sizeof(uint64_t)));
     {
         const long double PI = 3.141592653589793238462643383279502884L;
         for (int k = 0; k < 8; ++k) {
@@ -167,10 +162,19 @@ void Context::shutdown() {
     fgtw_.clear();
     if (maxerr) hipFree(maxerr);
     maxerr = nullptr;
+    if (words) hipFree(words);
+    words = nullptr;
+    held.clear();
     pool.release_all();
     hipStreamDestroy(stream);
     stream = nullptr;
     device = -1;
+}
+
+int Context::sync() {
+    if (hipStreamSynchronize(stream) != hipSuccess) return set_error(PG_EDEVICE, "device error");
+    held.clear();
+    return PG_OK;
 }
 
 void *Context::table_alloc(size_t bytes) {

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpgpu.so")
 
-PG_OK, PG_EINVAL, PG_ENOMEM, PG_EDEVICE, PG_ERANGE, PG_ENOINIT = 0, -1, -2, -3, -4, -5
+PG_OK, PG_EINVAL, PG_ENOMEM, PG_EDEVICE, PG_ERANGE, PG_ENOINIT, PG_EINTERNAL = 0, -1, -2, -3, -4, -5, -6
 PATH_AUTO, PATH_NTT, PATH_FFT, PATH_NTT_REF = 0, 1, 2, 3
 P64 = ctypes.POINTER(ctypes.c_uint64)
 SZ = ctypes.c_size_t
@@ -21,7 +21,8 @@ SZ = ctypes.c_size_t
 EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", "pg_buf_free",
            "pg_buf_capacity", "pg_buf_upload", "pg_buf_download", "pg_buf_ptr", "pg_mul_dev",
            "pg_mul_batch", "pg_batch_plan", "pg_batch_run", "pg_batch_errors", "pg_batch_free",
-           "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read"]
+           "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read",
+           "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev"]
 
 
 class Job(ctypes.Structure):
@@ -73,6 +74,11 @@ def load(path=None):
     L.pg_batch_run.argtypes = [ctypes.c_void_p]
     L.pg_batch_errors.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), SZ]
     L.pg_batch_free.argtypes = [ctypes.c_void_p]
+    L.pg_div.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, ctypes.POINTER(SZ), P64, SZ, P64, SZ]
+    L.pg_div_dev.argtypes = [ctypes.c_void_p, SZ, ctypes.POINTER(SZ), ctypes.c_void_p, SZ, ctypes.POINTER(SZ),
+                             ctypes.c_void_p, SZ, ctypes.c_void_p, SZ]
+    L.pg_recip.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, SZ]
+    L.pg_recip_dev.argtypes = [ctypes.c_void_p, SZ, ctypes.POINTER(SZ), ctypes.c_void_p, SZ, SZ]
     L.pg_timing_enable.argtypes = [ctypes.c_int]
     L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
     _lib = L
@@ -187,6 +193,40 @@ def timing_read():
     arr = (StageTime * max(n, 1))()
     n = L.pg_timing_read(arr, n)
     return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms, arr[i].bytes) for i in range(n)}
+
+
+def divmod_(a, b):
+    """(floor(a / b), a mod b) of limb arrays on the GPU (div_mulinv semantics:
+    b == 0 gives two empty arrays)."""
+    if not _inited:
+        init()
+    a, b = _trimmed(a), _trimmed(b)
+    q = np.zeros(max(len(a) - len(b) + 1, 1), np.uint64)
+    r = np.zeros(max(len(b), 1), np.uint64)
+    ql, rl = SZ(0), SZ(0)
+    _check(_lib.pg_div(_ptr(q), len(q), ctypes.byref(ql), _ptr(r), len(r), ctypes.byref(rl),
+                       _ptr(a), len(a), _ptr(b), len(b)))
+    return q[:ql.value].copy(), r[:rl.value].copy()
+
+
+def recip(b, bits):
+    """floor(2^bits / b) on the GPU (precn_reciprocal_newton semantics)."""
+    if not _inited:
+        init()
+    b = _trimmed(b)
+    x = np.zeros(max(bits // 64 + 2 - max(len(b) - 1, 0), 1), np.uint64)
+    xl = SZ(0)
+    _check(_lib.pg_recip(_ptr(x), len(x), ctypes.byref(xl), _ptr(b), len(b), bits))
+    return x[:xl.value].copy()
+
+
+def div_dev(q, r, a, an, b, bn):
+    """Device-resident division: q, r (DevBuf or None), a, b DevBufs.
+    Returns (q_len, r_len)."""
+    ql, rl = SZ(0), SZ(0)
+    _check(_lib.pg_div_dev(q.h if q else None, q.cap if q else 0, ctypes.byref(ql),
+                           r.h if r else None, r.cap if r else 0, ctypes.byref(rl), a.h, an, b.h, bn))
+    return ql.value, rl.value
 
 
 def mul_batch(pairs, path=PATH_AUTO, want_round=False):

@@ -16,9 +16,7 @@ def pytest_configure(config):
 
 def _ensure_oracle():
     lib = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not os.path.exists(lib):
-        subprocess.check_call(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile"),
-                               os.path.join(ROOT, "oracle", "liboracle.so")])
+    subprocess.check_call(["make", "-s", "-f", os.path.join(ROOT, "oracle", "Makefile"), lib])
     return lib
 
 
@@ -33,4 +31,11 @@ def oracle():
 def golden():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "products.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def golden_div():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "division.json")) as f:
         return json.load(f)

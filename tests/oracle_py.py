@@ -67,6 +67,12 @@ class Oracle:
         L.or_ntt_plan.restype = SZ
         L.or_ntt_plan.argtypes = [SZ, P64, SZ, P64, ctypes.POINTER(ctypes.c_int)]
         L.or_fill.argtypes = [P64, SZ, ctypes.c_uint64, ctypes.c_int]
+        L.or_divmod.restype = SZ
+        L.or_divmod.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, ctypes.POINTER(SZ), P64, SZ, P64, SZ]
+        L.or_recip.restype = SZ
+        L.or_recip.argtypes = [P64, SZ, P64, SZ, SZ]
+        L.or_sqrt.restype = SZ
+        L.or_sqrt.argtypes = [P64, SZ, P64, SZ]
 
     def _out(self, an, bn):
         return np.zeros(max(an + bn, 1), np.uint64)
@@ -139,6 +145,31 @@ class Oracle:
         p = ctypes.c_int(0)
         n = self.lib.or_ntt_plan(len(a), _p64(a), len(b), _p64(b), ctypes.byref(p))
         return n, p.value
+
+    def divmod(self, a, b):
+        """(floor(a / b), a mod b), trimmed."""
+        a, b = trim(a), trim(b)
+        q = np.zeros(max(len(a) - len(b) + 1, 1), np.uint64)
+        r = np.zeros(max(len(b), len(a), 1), np.uint64)
+        ql, rl = SZ(0), SZ(0)
+        st = self.lib.or_divmod(_p64(q), len(q), ctypes.byref(ql), _p64(r), len(r), ctypes.byref(rl),
+                                _p64(a), len(a), _p64(b), len(b))
+        assert st == 0
+        return q[:ql.value].copy(), r[:rl.value].copy()
+
+    def recip(self, b, bits):
+        b = trim(b)
+        x = np.zeros(bits // 64 + 2, np.uint64)
+        n = self.lib.or_recip(_p64(x), len(x), _p64(b), len(b), bits)
+        assert n != 2**64 - 1
+        return x[:n].copy()
+
+    def sqrt(self, a):
+        a = trim(a)
+        x = np.zeros(len(a) // 2 + 3, np.uint64)
+        n = self.lib.or_sqrt(_p64(x), len(x), _p64(a), len(a))
+        assert n != 2**64 - 1
+        return x[:n].copy()
 
     def fill(self, n, seed, dist):
         a = np.zeros(n, np.uint64)
