@@ -137,6 +137,29 @@ int pg_div_dev(pg_buf *q, size_t q_cap, size_t *q_len, pg_buf *r, size_t r_cap, 
 int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t bn, size_t bits);
 int pg_recip_dev(pg_buf *x, size_t x_cap, size_t *x_len, const pg_buf *b, size_t bn, size_t bits);
 
+/* precn_sqrt (src/sqrt.cpp:71-86, prec.hpp:203): floor(sqrt(a)); x needs
+ * room for an / 2 + 1 limbs.  Every division and square runs on the GPU. */
+int pg_sqrt(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *a, size_t an);
+
+/* precn_base_convert(a, 10, ...) (src/base_convert.cpp:107-172, prec.hpp:205)
+ * as text: the decimal digits of a, most significant first, no leading
+ * zeros and no terminator; *len = 0 for a == 0.  out needs room for
+ * an * 19.27 + 1 characters. */
+int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t an);
+
+/* chud_bs(0, nterms, need_p) (pi/pi_chudnovsky.cpp:117-148): P, Q and T of
+ * the Chudnovsky series over terms [0, nterms), nterms <= 2^20.  T is
+ * returned as its magnitude (it is positive for every nterms >= 1).  p may
+ * be NULL (need_p = false, as the root call of pi_digits, line 168). */
+int pg_chud_split(uint64_t *p, size_t p_cap, size_t *p_len, uint64_t *q, size_t q_cap, size_t *q_len,
+                  uint64_t *t, size_t t_cap, size_t *t_len, size_t nterms);
+
+/* The pi program, pi_digits (pi/pi_chudnovsky.cpp:162-193): "3." followed by
+ * `digits` digits (the text its --file output holds, without the final
+ * newline).  phase_ms, if not NULL, receives 4 wall times in ms:
+ * binary_split, sqrt_scale, final_div, to_decimal (the --phases lines). */
+int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms);
+
 /* Waits for all queued device work. */
 int pg_sync(void);
 

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpgpu.so")
 SOURCES = ["pg_ntt.hip", "pg_fft.hip", "pg_small.hip", "pg_carry.hip", "pg_batch.hip", "pg_runtime.hip", "pg_dint.hip",
-           "pg_api.hip"]
+           "pg_pi.hip", "pg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -3,6 +3,7 @@
 #include <cstring>
 #include "pg_batch.h"
 #include "pg_dint.h"
+#include "pg_pi.h"
 #include "pg_runtime.h"
 #include "pgpu.h"
 
@@ -319,6 +320,69 @@ int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t
     int st = recip_exact(dx, xl, db.d, bn, bits);
     if (st) return st;
     return finish_out(dx, xl, x, x_cap, x_len, false);
+}
+
+// ---- square root, decimal conversion, binary splitting, pi -------------------------
+
+int pg_sqrt(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *a, size_t an) {
+    LOCK;
+    NEED_INIT;
+    if (an && !a) return set_error(PG_EINVAL, "pg_sqrt: bad argument");
+    an = host_trim(a, an);
+    Context &C = ctx();
+    DBuf da(an), dx;
+    if (!da.d) return set_error(PG_ENOMEM, "pg_sqrt: operand");
+    if (an) HIP_TRY(hipMemcpyAsync(da.d, a, an * 8, hipMemcpyHostToDevice, C.stream));
+    size_t xl = 0;
+    int st = isqrt(dx, xl, da.d, an);
+    if (st) return st;
+    return finish_out(dx, xl, x, x_cap, x_len, false);
+}
+
+static int put_string(const std::string &s, char *out, size_t cap, size_t *len) {
+    if (len) *len = s.size();
+    if (!out) return PG_OK;
+    if (cap < s.size()) return set_error(PG_EINVAL, "output capacity too small");
+    std::memcpy(out, s.data(), s.size());
+    return PG_OK;
+}
+
+int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t an) {
+    LOCK;
+    NEED_INIT;
+    if (an && !a) return set_error(PG_EINVAL, "pg_to_decimal: bad argument");
+    an = host_trim(a, an);
+    Context &C = ctx();
+    DBuf da(an);
+    if (!da.d) return set_error(PG_ENOMEM, "pg_to_decimal: operand");
+    if (an) HIP_TRY(hipMemcpyAsync(da.d, a, an * 8, hipMemcpyHostToDevice, C.stream));
+    std::string s;
+    int st = to_decimal(da.d, an, s);
+    if (st) return st;
+    return put_string(s, out, cap, len);
+}
+
+int pg_chud_split(uint64_t *p, size_t p_cap, size_t *p_len, uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *t,
+                  size_t t_cap, size_t *t_len, size_t nterms) {
+    LOCK;
+    NEED_INIT;
+    std::vector<DBuf> keep;
+    DBuf P, Q, T;
+    size_t lp = 0, lq = 0, lt = 0;
+    int st = chud_split(nterms, P, lp, Q, lq, T, lt, p != nullptr, keep);
+    if (st) return st;
+    if ((st = finish_out(P, lp, p, p_cap, p_len, false))) return st;
+    if ((st = finish_out(Q, lq, q, q_cap, q_len, false))) return st;
+    return finish_out(T, lt, t, t_cap, t_len, false);
+}
+
+int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms) {
+    LOCK;
+    NEED_INIT;
+    std::string s;
+    int st = pi_digits(digits, s, phase_ms);
+    if (st) return st;
+    return put_string(s, out, cap, len);
 }
 
 int pg_timing_enable(int on) {

@@ -1,0 +1,816 @@
+// pg_pi.hip -- Chudnovsky binary splitting, integer square root, decimal
+// conversion and the pi program (pi/pi_chudnovsky.cpp) on device operands.
+//
+// Binary splitting (pi_chudnovsky.cpp:117-148).  P, Q and T of a term range
+// do not depend on where the range is split, so the tree is built level by
+// level over aligned power-of-two blocks instead of the reference's
+// recursion.  Levels 1-5 run one thread per node (operands of at most a few
+// dozen limbs, fixed slots); above that every level is one batched product
+// launch plus one linear carry launch for T.  Sign of T(a, b) is (-1)^a:
+// consecutive terms shrink by about 10^14, so the first term dominates.
+// Aligned blocks therefore have T >= 0 from level 1 on, the only
+// subtraction is in the level-1 merge, and every later T is a sum.
+//
+// Square root: the divide-and-refine recursion of src/sqrt.cpp:62-86.
+// Decimal conversion: divide and conquer by W_i = (10^19)^(2^i) as
+// src/base_convert.cpp:56-105, with one exact reciprocal per level shared by
+// every node and a one-step quotient correction, then one thread per block
+// of 32 chunks of 19 digits.
+#include "pg_pi.h"
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include "pg_batch.h"
+#include "pgpu.h"
+
+namespace pg {
+
+constexpr uint64_t CHUD_C3_OVER_24 = 10939058860032000ull;   // pi_chudnovsky.cpp:11
+constexpr uint64_t CHUD_A = 13591409u, CHUD_B = 545140134u;   // 12-13
+constexpr int CHUD_THREAD_LEVELS = 5;                         // per-thread levels 1..5
+constexpr uint64_t TEN19 = 10000000000000000000ull;
+constexpr int DEC_LEAF = 32;                                  // chunks per conversion thread
+
+// ---- per-thread integers -----------------------------------------------------------
+
+__device__ __forceinline__ void mac(uint64_t a, uint64_t b, uint64_t add, uint64_t &c, uint64_t &out) {
+    uint64_t lo = a * b, hi = __umul64hi(a, b);
+    lo += c;
+    hi += lo < c;
+    lo += add;
+    hi += lo < add;
+    out = lo;
+    c = hi;
+}
+
+// out[0, an + bn) = a * b
+__device__ void t_mul(uint64_t *out, const uint64_t *a, uint32_t an, const uint64_t *b, uint32_t bn) {
+    for (uint32_t k = 0; k < an + bn; ++k) out[k] = 0;
+    for (uint32_t i = 0; i < an; ++i) {
+        uint64_t c = 0;
+        for (uint32_t j = 0; j < bn; ++j) mac(a[i], b[j], out[i + j], c, out[i + j]);
+        out[i + bn] = c;
+    }
+}
+
+// out[0, on) += a * b or -= a * b (the result must fit in on limbs and stay >= 0)
+template <bool SUB>
+__device__ void t_addmul(uint64_t *out, uint32_t on, const uint64_t *a, uint32_t an, const uint64_t *b,
+                         uint32_t bn) {
+    for (uint32_t i = 0; i < an; ++i) {
+        uint64_t c = 0;
+        uint32_t k = i;
+        for (uint32_t j = 0; j < bn; ++j, ++k) {
+            uint64_t p;
+            mac(a[i], b[j], 0, c, p);
+            if (SUB) {
+                const uint64_t o = out[k];
+                out[k] = o - p;
+                c += o < p;
+            } else {
+                const uint64_t s = out[k] + p;
+                c += s < p;
+                out[k] = s;
+            }
+        }
+        for (; c && k < on; ++k) {
+            const uint64_t o = out[k];
+            if (SUB) {
+                out[k] = o - c;
+                c = o < c;
+            } else {
+                out[k] = o + c;
+                c = out[k] < c;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t t_len(const uint64_t *a, uint32_t n) {
+    while (n && !a[n - 1]) --n;
+    return n;
+}
+
+// Leaf k (pi_chudnovsky.cpp:118-135): p, q, |t| as two limbs each.
+__device__ void chud_leaf(uint64_t k, uint64_t p[2], uint64_t q[2], uint64_t t[2]) {
+    if (k == 0) {
+        p[0] = 1; p[1] = 0;
+        q[0] = 1; q[1] = 0;
+        t[0] = CHUD_A; t[1] = 0;
+        return;
+    }
+    const uint64_t x = (6 * k - 5) * (2 * k - 1);   // < 2^43 for k < 2^20
+    p[0] = x * (6 * k - 1);
+    p[1] = __umul64hi(x, 6 * k - 1);
+    const uint64_t k3 = k * k * k;                  // < 2^60
+    q[0] = k3 * CHUD_C3_OVER_24;
+    q[1] = __umul64hi(k3, CHUD_C3_OVER_24);
+    const uint64_t m = CHUD_B * k + CHUD_A;         // < 2^50
+    uint64_t c = 0;
+    mac(p[0], m, 0, c, t[0]);
+    t[1] = p[1] * m + c;                            // |t| < 2^115
+}
+
+struct ChudLevel {
+    uint64_t *P, *Q, *T;     // node i at + i * slot
+    uint32_t *lp, *lq, *lt;
+    uint32_t slot;
+    uint32_t count;
+};
+
+// Level 1 from the leaves: node i merges terms 2i (even, t >= 0) and 2i + 1
+// (odd, t <= 0): Q = q_a q_b, T = |t_a| q_b - |t_b| p_a, P = p_a p_b.
+__global__ void __launch_bounds__(256) k_chud_base(uint64_t nterms, ChudLevel out) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= out.count) return;
+    uint64_t pa[2], qa[2], ta[2];
+    chud_leaf(2 * i, pa, qa, ta);
+    uint64_t *P = out.P + i * out.slot, *Q = out.Q + i * out.slot, *T = out.T + i * out.slot;
+    if (2 * i + 1 >= nterms) {
+        for (int k = 0; k < 2; ++k) { P[k] = pa[k]; Q[k] = qa[k]; T[k] = ta[k]; }
+        out.lp[i] = t_len(P, 2); out.lq[i] = t_len(Q, 2); out.lt[i] = t_len(T, 2);
+        return;
+    }
+    uint64_t pb[2], qb[2], tb[2];
+    chud_leaf(2 * i + 1, pb, qb, tb);
+    t_mul(Q, qa, 2, qb, 2);
+    t_mul(P, pa, 2, pb, 2);
+    t_mul(T, ta, 2, qb, 2);
+    t_addmul<true>(T, 4, tb, 2, pa, 2);
+    out.lp[i] = t_len(P, 4); out.lq[i] = t_len(Q, 4); out.lt[i] = t_len(T, 4);
+}
+
+// Level j -> j + 1 (both children have T >= 0): Q = lq rq, T = lt rq + rt lp,
+// P = lp rp; an unpaired last node moves up unchanged.
+__global__ void __launch_bounds__(256) k_chud_merge(ChudLevel in, ChudLevel out) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= out.count) return;
+    const uint64_t l = 2 * i, r = l + 1;
+    uint64_t *P = out.P + i * out.slot, *Q = out.Q + i * out.slot, *T = out.T + i * out.slot;
+    const uint64_t *lP = in.P + l * in.slot, *lQ = in.Q + l * in.slot, *lT = in.T + l * in.slot;
+    const uint32_t llp = in.lp[l], llq = in.lq[l], llt = in.lt[l];
+    if (r >= in.count) {
+        for (uint32_t k = 0; k < llp; ++k) P[k] = lP[k];
+        for (uint32_t k = 0; k < llq; ++k) Q[k] = lQ[k];
+        for (uint32_t k = 0; k < llt; ++k) T[k] = lT[k];
+        out.lp[i] = llp; out.lq[i] = llq; out.lt[i] = llt;
+        return;
+    }
+    const uint64_t *rP = in.P + r * in.slot, *rQ = in.Q + r * in.slot, *rT = in.T + r * in.slot;
+    const uint32_t rlp = in.lp[r], rlq = in.lq[r], rlt = in.lt[r];
+    t_mul(Q, lQ, llq, rQ, rlq);
+    t_mul(P, lP, llp, rP, rlp);
+    const uint32_t tn = (llt + rlq > rlt + llp ? llt + rlq : rlt + llp) + 1;
+    t_mul(T, lT, llt, rQ, rlq);
+    for (uint32_t k = llt + rlq; k < tn; ++k) T[k] = 0;
+    t_addmul<false>(T, tn, rT, rlt, lP, llp);
+    out.lp[i] = t_len(P, llp + rlp); out.lq[i] = t_len(Q, llq + rlq); out.lt[i] = t_len(T, tn);
+}
+
+struct LenReq {
+    const uint64_t *p;
+    uint32_t n;
+};
+__global__ void k_lengths(const LenReq *req, uint32_t n, uint32_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = req[i].n;
+    while (k && !req[i].p[k - 1]) --k;
+    out[i] = k;
+}
+
+// ---- binary splitting ------------------------------------------------------------
+
+namespace {
+
+struct Node {
+    const uint64_t *P = nullptr, *Q = nullptr, *T = nullptr;
+    size_t lp = 0, lq = 0, lt = 0;
+};
+
+unsigned blocks_for(size_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// Uploads a host vector through the stream (image kept alive until sync).
+template <class T>
+int upload(DBuf &dst, const std::vector<T> &v) {
+    const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 8);
+    if (!dst.reset((bytes + 7) / 8)) return set_error(PG_ENOMEM, "descriptor upload");
+    std::vector<char> img(bytes, 0);
+    if (!v.empty()) std::memcpy(img.data(), v.data(), v.size() * sizeof(T));
+    HIP_TRY(hipMemcpyAsync(dst.d, img.data(), bytes, hipMemcpyHostToDevice, ctx().stream));
+    ctx().hold(std::move(img));
+    return PG_OK;
+}
+
+// Trimmed lengths of many device arrays (one launch, one read).
+int lengths(const std::vector<LenReq> &req, std::vector<uint32_t> &out) {
+    Context &C = ctx();
+    out.assign(req.size(), 0);
+    if (req.empty()) return PG_OK;
+    DBuf dreq, dout((req.size() * 4 + 7) / 8);
+    int st = upload(dreq, req);
+    if (st) return st;
+    if (!dout.d) return set_error(PG_ENOMEM, "lengths");
+    hipLaunchKernelGGL(k_lengths, dim3(blocks_for(req.size())), dim3(256), 0, C.stream, (const LenReq *)dreq.d,
+                       (uint32_t)req.size(), (uint32_t *)dout.d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out.data(), dout.d, req.size() * 4, hipMemcpyDeviceToHost, C.stream));
+    return C.sync();
+}
+
+}  // namespace
+
+int chud_split(size_t nterms, DBuf &Pout, size_t &lp, DBuf &Qout, size_t &lq, DBuf &Tout, size_t &lt,
+               bool need_p, std::vector<DBuf> &keep) {
+    Context &C = ctx();
+    lp = lq = lt = 0;
+    if (!nterms) return set_error(PG_EINVAL, "binary splitting needs at least one term");
+    if (nterms > ((size_t)1 << 20)) return set_error(PG_ERANGE, "binary splitting limited to 2^20 terms");
+    // per-thread levels
+    ChudLevel lv[CHUD_THREAD_LEVELS + 1]{};
+    size_t count = (nterms + 1) / 2;
+    int st;
+    for (int j = 1; j <= CHUD_THREAD_LEVELS; ++j) {
+        ChudLevel &L = lv[j];
+        L.count = (uint32_t)count;
+        L.slot = 3u << j;
+        keep.emplace_back(3 * count * L.slot);
+        keep.emplace_back((3 * count * 4 + 7) / 8);
+        if (!keep[keep.size() - 2].d || !keep.back().d) return set_error(PG_ENOMEM, "binary splitting");
+        uint64_t *base = keep[keep.size() - 2].d;
+        uint32_t *lens = (uint32_t *)keep.back().d;
+        L.P = base; L.Q = base + count * L.slot; L.T = base + 2 * count * L.slot;
+        L.lp = lens; L.lq = lens + count; L.lt = lens + 2 * count;
+        hipEvent_t e0 = nullptr;
+        C.stage_begin(e0);
+        if (j == 1)
+            hipLaunchKernelGGL(k_chud_base, dim3(blocks_for(count)), dim3(256), 0, C.stream, (uint64_t)nterms, L);
+        else
+            hipLaunchKernelGGL(k_chud_merge, dim3(blocks_for(count)), dim3(256), 0, C.stream, lv[j - 1], L);
+        C.stage_end("chud_thread", e0, 0.0);
+        HIP_TRY(hipGetLastError());
+        if (count == 1 && j < CHUD_THREAD_LEVELS) {   // done early: carry the root up
+            lv[CHUD_THREAD_LEVELS] = L;
+            break;
+        }
+        count = (count + 1) / 2;
+    }
+    // nodes of the last per-thread level
+    const ChudLevel &top = lv[CHUD_THREAD_LEVELS];
+    std::vector<uint32_t> lens(3 * (size_t)top.count);
+    HIP_TRY(hipMemcpyAsync(lens.data(), top.lp, lens.size() * 4, hipMemcpyDeviceToHost, C.stream));
+    if ((st = C.sync())) return st;
+    std::vector<Node> nodes(top.count);
+    for (size_t i = 0; i < top.count; ++i) {
+        Node &n = nodes[i];
+        n.P = top.P + i * top.slot; n.Q = top.Q + i * top.slot; n.T = top.T + i * top.slot;
+        n.lp = lens[i]; n.lq = lens[top.count + i]; n.lt = lens[2 * top.count + i];
+    }
+    // batched levels
+    while (nodes.size() > 1) {
+        const size_t nm = nodes.size() / 2, np = (nodes.size() + 1) / 2;
+        size_t total = 0;
+        for (size_t k = 0; k < nm; ++k) {
+            const Node &l = nodes[2 * k], &r = nodes[2 * k + 1];
+            const bool wantp = need_p || k + 1 < np;
+            total += (wantp ? l.lp + r.lp : 0) + l.lq + r.lq + 2 * (l.lt + r.lq) + 2 * (r.lt + l.lp) + 1;
+        }
+        keep.emplace_back(total + 1);
+        DBuf &arena = keep.back();
+        if (!arena.d) return set_error(PG_ENOMEM, "binary splitting");
+        std::vector<pg_job> jobs;
+        std::vector<CarryNode> lin;
+        std::vector<Node> up(np);
+        std::vector<size_t> bound(3 * np);
+        uint64_t *cur = arena.d;
+        auto job = [&](const uint64_t *a, size_t an, const uint64_t *b, size_t bn) {
+            uint64_t *r = cur;
+            cur += an + bn;
+            if (an && bn) {
+                pg_job j{};
+                j.a = a; j.an = an; j.b = b; j.bn = bn; j.r = r; j.r_cap = an + bn; j.path = PG_PATH_AUTO;
+                jobs.push_back(j);
+            } else {
+                lin.push_back(lin_node(r, std::max<size_t>(an + bn, 1)));   // zero fill
+            }
+            return r;
+        };
+        for (size_t k = 0; k < nm; ++k) {
+            const Node &l = nodes[2 * k], &r = nodes[2 * k + 1];
+            const bool wantp = need_p || k + 1 < np;
+            Node &u = up[k];
+            if (wantp) u.P = job(l.P, l.lp, r.P, r.lp);
+            u.Q = job(l.Q, l.lq, r.Q, r.lq);
+            const uint64_t *t1 = job(l.T, l.lt, r.Q, r.lq);
+            const uint64_t *t2 = job(r.T, r.lt, l.P, l.lp);
+            const size_t tn = std::max(l.lt + r.lq, r.lt + l.lp) + 1;
+            uint64_t *t = cur;
+            cur += tn;
+            CarryNode c = lin_node(t, tn);
+            c.x = t1; c.xn = l.lt + r.lq; c.y = t2; c.yn = r.lt + l.lp;
+            lin.push_back(c);
+            u.T = t;
+            bound[3 * k] = wantp ? l.lp + r.lp : 0; bound[3 * k + 1] = l.lq + r.lq; bound[3 * k + 2] = tn;
+        }
+        if (np > nm) {
+            up[nm] = nodes.back();
+            bound[3 * nm] = up[nm].lp; bound[3 * nm + 1] = up[nm].lq; bound[3 * nm + 2] = up[nm].lt;
+        }
+        // the zero-fill nodes must run before the sums read them: products
+        // of an empty operand are written by the fills, the sums come after
+        std::vector<CarryNode> fills, sums;
+        for (auto &c : lin) (c.x ? sums : fills).push_back(c);
+        if (!fills.empty() && (st = lin_run(fills.data(), fills.size(), "chud_sum"))) return st;
+        if (!jobs.empty()) {
+            Batch b;
+            b.use_graph = false;
+            if ((st = b.build(jobs.data(), jobs.size()))) return st;
+            if ((st = b.run())) return st;
+        }
+        if ((st = lin_run(sums.data(), sums.size(), "chud_sum"))) return st;
+        std::vector<LenReq> req;
+        for (size_t k = 0; k < np; ++k) {
+            req.push_back({up[k].P, (uint32_t)bound[3 * k]});
+            req.push_back({up[k].Q, (uint32_t)bound[3 * k + 1]});
+            req.push_back({up[k].T, (uint32_t)bound[3 * k + 2]});
+        }
+        std::vector<uint32_t> got;
+        if ((st = lengths(req, got))) return st;
+        for (size_t k = 0; k < np; ++k) {
+            up[k].lp = up[k].P ? got[3 * k] : 0;
+            up[k].lq = got[3 * k + 1];
+            up[k].lt = got[3 * k + 2];
+        }
+        nodes.swap(up);
+    }
+    const Node &root = nodes[0];
+    lp = root.lp; lq = root.lq; lt = root.lt;
+    if (!Pout.reset(std::max<size_t>(lp, 1)) || !Qout.reset(std::max<size_t>(lq, 1)) ||
+        !Tout.reset(std::max<size_t>(lt, 1)))
+        return set_error(PG_ENOMEM, "binary splitting");
+    if ((st = dcopy(Pout.d, root.P, lp)) || (st = dcopy(Qout.d, root.Q, lq)) || (st = dcopy(Tout.d, root.T, lt)))
+        return st;
+    return C.sync();
+}
+
+// ---- integer square root ----------------------------------------------------------
+
+namespace {
+
+std::vector<uint64_t> small_isqrt(std::vector<uint64_t> a) {
+    // digit-by-digit binary square root on at most a few limbs
+    auto cmp = [](const std::vector<uint64_t> &x, const std::vector<uint64_t> &y) {
+        for (size_t i = std::max(x.size(), y.size()); i-- > 0;) {
+            const uint64_t u = i < x.size() ? x[i] : 0, v = i < y.size() ? y[i] : 0;
+            if (u != v) return u < v ? -1 : 1;
+        }
+        return 0;
+    };
+    const size_t n = a.size() + 1;
+    a.resize(n, 0);
+    std::vector<uint64_t> res(n, 0), bit(n, 0), t(n, 0);
+    size_t hb = 0;
+    for (size_t i = 0; i < 64 * n; ++i)
+        if ((a[i / 64] >> (i % 64)) & 1) hb = i;
+    hb &= ~(size_t)1;
+    bit[hb / 64] = 1ull << (hb % 64);
+    for (;;) {
+        uint64_t c = 0;
+        for (size_t i = 0; i < n; ++i) {   // t = res + bit
+            const uint64_t s = res[i] + bit[i];
+            const uint64_t s2 = s + c;
+            c = (s < res[i]) | (s2 < s);
+            t[i] = s2;
+        }
+        const bool ge = cmp(a, t) >= 0;
+        if (ge) {
+            uint64_t br = 0;
+            for (size_t i = 0; i < n; ++i) {
+                const uint64_t d = a[i] - t[i] - br;
+                br = (a[i] < t[i]) || (a[i] - t[i] < br);
+                a[i] = d;
+            }
+        }
+        for (size_t i = 0; i < n; ++i) res[i] = (res[i] >> 1) | (i + 1 < n ? res[i + 1] << 63 : 0);
+        if (ge)
+            for (size_t i = 0; i < n; ++i) {   // res += bit
+                const uint64_t s = res[i] + bit[i];
+                const uint64_t cc = s < res[i];
+                res[i] = s;
+                for (size_t k = i + 1; cc && k < n; ++k)
+                    if (++res[k]) break;
+            }
+        bool zero = true;
+        for (size_t i = 0; i < n; ++i) {
+            bit[i] = (bit[i] >> 2) | (i + 1 < n ? bit[i + 1] << 62 : 0);
+            zero &= bit[i] == 0;
+        }
+        if (zero) break;
+    }
+    while (!res.empty() && !res.back()) res.pop_back();
+    return res;
+}
+
+// sqrt_refine_upper (sqrt.cpp:41-60) from an upper bound x (xl limbs).
+int refine_upper(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
+    int st;
+    for (int it = 0; it < 64; ++it) {   // y = (x + a / x) >> 1 until y >= x
+        size_t xt = 0;
+        if ((st = top_len(x.d, xl, &xt))) return st;
+        DBuf q;
+        size_t ql = 0;
+        if ((st = div_mulinv(q, ql, nullptr, nullptr, a, an, x.d, xt))) return st;
+        const size_t sm = std::max(xt, ql) + 1;
+        DBuf s(sm), y(sm);
+        if (!s.d || !y.d) return set_error(PG_ENOMEM, "sqrt");
+        CarryNode c = lin_node(s.d, sm);
+        c.x = x.d; c.xn = xt; c.y = q.d; c.yn = ql;
+        if ((st = lin_one(c, "sqrt"))) return st;
+        CarryNode h = lin_node(y.d, sm);
+        h.x = s.d; h.xn = sm; h.xsh = 1;
+        if ((st = lin_one(h, "sqrt"))) return st;
+        DBuf dif(sm + 1);
+        CarryNode d = lin_node(dif.d, sm + 1);
+        d.x = y.d; d.xn = sm; d.y = x.d; d.yn = xt; d.yneg = 1; d.cin = 1; d.flag = ctx().words + 16;
+        if ((st = lin_one(d, "sqrt"))) return st;
+        uint64_t f = 0;
+        if ((st = read_words(ctx().words + 16, 1, &f))) return st;
+        if (f) {   // y >= x: x is the floor root unless the bound was not one
+            xl = xt;
+            break;
+        }
+        x = std::move(y);
+        xl = sm;
+    }
+    // r = a - x^2, then the two loops of sqrt.cpp:48-58
+    const size_t rm = std::max(an, 2 * xl) + 2;
+    DBuf sq(2 * xl), r(rm), r2(rm), x2(xl + 1);
+    if (!sq.d || !r.d || !r2.d || !x2.d) return set_error(PG_ENOMEM, "sqrt");
+    if ((st = dmul(sq.d, x.d, xl, x.d, xl))) return st;
+    uint64_t *flag = ctx().words + 17;
+    CarryNode c = lin_node(r.d, rm);
+    c.x = a; c.xn = an; c.y = sq.d; c.yn = 2 * xl; c.yneg = 1; c.cin = 1; c.flag = flag;
+    if ((st = lin_one(c, "sqrt"))) return st;
+    DBuf xx(xl + 1);
+    if (!xx.d || (st = dset(xx.d, xl + 1, nullptr, 0)) || (st = dcopy(xx.d, x.d, xl))) return st ? st : PG_ENOMEM;
+    size_t xn = xl + 1;
+    for (int it = 0; it < 64; ++it) {
+        uint64_t f = 0;
+        if ((st = read_words(flag, 1, &f))) return st;
+        if (f == 0) {   // x^2 > a: x -= 1, r += 2 x_new + 1
+            CarryNode m = lin_node(x2.d, xn);
+            m.x = xx.d; m.xn = xn; m.yneg = 1;
+            if ((st = lin_one(m, "sqrt"))) return st;
+            CarryNode u = lin_node(r2.d, rm);
+            u.x = r.d; u.xn = rm; u.y = x2.d; u.yn = xn; u.ysh = -1; u.cin = 1; u.flag = flag;
+            if ((st = lin_one(u, "sqrt"))) return st;
+        } else {        // (x + 1)^2 <= a  <=>  r >= 2 x + 1
+            CarryNode u = lin_node(r2.d, rm);
+            u.x = r.d; u.xn = rm; u.y = xx.d; u.yn = xn; u.ysh = -1; u.yneg = 1; u.flag = flag;
+            if ((st = lin_one(u, "sqrt"))) return st;
+            uint64_t g = 0;
+            if ((st = read_words(flag, 1, &g))) return st;
+            if (g == 0) {
+                x = std::move(xx);
+                return top_len(x.d, xn, &xl);
+            }
+            CarryNode m = lin_node(x2.d, xn);
+            m.x = xx.d; m.xn = xn; m.cin = 1;
+            if ((st = lin_one(m, "sqrt"))) return st;
+        }
+        std::swap(xx, x2);
+        std::swap(r, r2);
+    }
+    return set_error(PG_EINTERNAL, "square root correction did not converge");
+}
+
+}  // namespace
+
+int isqrt(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
+    xl = 0;
+    int st;
+    if (!an) return PG_OK;
+    if (an <= 4) {   // sqrt.cpp:78: small operands from the top seed
+        std::vector<uint64_t> h(an);
+        if ((st = read_words(a, an, h.data()))) return st;
+        const std::vector<uint64_t> r = small_isqrt(h);
+        xl = r.size();
+        if (!x.reset(std::max<size_t>(xl, 1))) return set_error(PG_ENOMEM, "sqrt");
+        return dset(x.d, std::max<size_t>(xl, 1), r.data(), r.size());
+    }
+    const size_t drop = (an / 2) & ~(size_t)1;   // sqrt.cpp:80
+    DBuf h;
+    size_t hl = 0;
+    if ((st = isqrt(h, hl, a + drop, an - drop))) return st;
+    // upper = (high_root + 1) << (32 drop)   (sqrt.cpp:84)
+    xl = hl + 1 + drop / 2;
+    if (!x.reset(xl)) return set_error(PG_ENOMEM, "sqrt");
+    CarryNode c = lin_node(x.d, xl);
+    c.x = h.d; c.xn = hl; c.xsh = -(int64_t)(32 * drop); c.pow2 = (int64_t)(32 * drop);   // (h << s) + 2^s
+    if ((st = lin_one(c, "sqrt"))) return st;
+    return refine_upper(x, xl, a, an);
+}
+
+// ---- decimal conversion ------------------------------------------------------------
+
+struct DecLeaf {
+    const uint64_t *p;
+    uint32_t n;          // <= DEC_LEAF limbs, value < 10^(19 DEC_LEAF)
+    uint64_t chunk0;     // index of its lowest 19-digit chunk
+};
+
+// Division of (u1, u0) by the normalized invariant d with v = floor((2^128 - 1) / d) - 2^64:
+// Moller and Granlund, "Improved division by invariant integers" (IEEE TC 60(2), 2011), Alg. 4.
+__device__ __forceinline__ uint64_t div2by1(uint64_t u1, uint64_t u0, uint64_t d, uint64_t v, uint64_t &r) {
+    uint64_t q0 = v * u1, q1 = __umul64hi(v, u1);
+    const uint64_t s = q0 + u0;
+    q1 += u1 + (s < q0);
+    q0 = s;
+    q1 += 1;
+    r = u0 - q1 * d;
+    if (r > q0) {
+        q1 -= 1;
+        r += d;
+    }
+    if (r >= d) {
+        q1 += 1;
+        r -= d;
+    }
+    return q1;
+}
+
+// One thread per leaf: DEC_LEAF chunks of 19 digits into out (most
+// significant digit first; total length dcap).
+__global__ void __launch_bounds__(64) k_dec_leaf(const DecLeaf *leaves, uint32_t n, uint64_t v, char *out,
+                                                 uint64_t dcap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DecLeaf L = leaves[i];
+    uint64_t y[DEC_LEAF];
+#pragma unroll
+    for (int k = 0; k < DEC_LEAF; ++k) y[k] = k < (int)L.n ? L.p[k] : 0;
+    for (int c = 0; c < DEC_LEAF; ++c) {
+        uint64_t r = 0;
+#pragma unroll
+        for (int k = DEC_LEAF - 1; k >= 0; --k) y[k] = div2by1(r, y[k], TEN19, v, r);
+        const uint64_t pos = (L.chunk0 + c) * 19;
+        uint64_t x = r;
+        for (int t = 0; t < 19; ++t) {
+            const uint64_t q = __umul64hi(x, 0xCCCCCCCCCCCCCCCDull) >> 3;   // x / 10
+            out[dcap - 1 - (pos + t)] = (char)('0' + (x - q * 10));
+            x = q;
+        }
+    }
+}
+
+__global__ void k_fill_chars(char *p, uint64_t n, char c) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = c;
+}
+
+namespace {
+size_t bits_of_top(uint64_t top, size_t limbs) { return limbs ? (limbs - 1) * 64 + 64 - (size_t)__builtin_clzll(top) : 0; }
+}  // namespace
+
+int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
+    Context &C = ctx();
+    out.clear();
+    int st;
+    size_t xb = 0;
+    if ((st = top_len(x, xn, &xn))) return st;
+    if (!xn) return PG_OK;
+    if ((st = bit_length(x, xn, &xb))) return st;
+    // chunks of 19 digits: 2^L >= ceil(digits / 19) with digits <= bits * log10(2) + 1
+    const size_t maxdig = (size_t)((double)xb * 0.30103) + 2;
+    const size_t nchunk = (maxdig + 18) / 19;
+    int L = 0;
+    while (((size_t)1 << L) < nchunk) ++L;
+    const int leaf_log = 5;   // DEC_LEAF = 2^5
+    const int top = std::max(L, leaf_log);
+    // W_i = (10^19)^(2^i), i < top; V_i = floor(2^(2 bits(W_i)) / W_i) for i >= leaf_log
+    std::vector<DBuf> W(top), V(top);
+    std::vector<size_t> wl(top), wb(top), vl(top);
+    {
+        const uint64_t ten19 = TEN19;
+        if (!W[0].reset(1) || (st = dset(W[0].d, 1, &ten19, 1))) return st ? st : set_error(PG_ENOMEM, "powers");
+        wl[0] = 1;
+        wb[0] = 64;
+        for (int i = 1; i < top; ++i) {
+            if (!W[i].reset(2 * wl[i - 1])) return set_error(PG_ENOMEM, "powers");
+            if ((st = dmul(W[i].d, W[i - 1].d, wl[i - 1], W[i - 1].d, wl[i - 1]))) return st;
+            if ((st = top_len(W[i].d, 2 * wl[i - 1], &wl[i])) || (st = bit_length(W[i].d, wl[i], &wb[i]))) return st;
+        }
+        for (int i = leaf_log; i < top; ++i)
+            if ((st = recip_exact(V[i], vl[i], W[i].d, wl[i], 2 * wb[i]))) return st;
+    }
+    struct DNode {
+        const uint64_t *p;
+        size_t n, bits, chunk0;
+    };
+    std::vector<DNode> nodes{{x, xn, xb, 0}};
+    std::vector<DBuf> keep;
+    DBuf flags;
+    std::vector<size_t> check_flags;   // indices of words that must read 1 (ok) / 0
+    for (int i = top - 1; i >= leaf_log; --i) {
+        const size_t nw = wl[i], vb = vl[i], sb = 2 * wb[i];
+        std::vector<DNode> next;
+        std::vector<size_t> jobs_of;   // node index of each division
+        for (size_t k = 0; k < nodes.size(); ++k) {
+            if (nodes[k].bits + 1 <= wb[i]) next.push_back(nodes[k]);   // value < 2^(wb-1) <= W_i
+            else jobs_of.push_back(k);
+        }
+        const size_t nj = jobs_of.size();
+        if (!nj) {
+            nodes.swap(next);
+            continue;
+        }
+        // per job: ax (n + vb), q (ql), p (ql + nw), r (nw + 1), t (nw + 1), r2 (nw + 1), q2 (ql)
+        std::vector<size_t> qlen(nj), off(nj);
+        size_t total = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            const DNode &y = nodes[jobs_of[j]];
+            qlen[j] = limbs_for_bits(y.bits + 2 - wb[i]) + 1;
+            off[j] = total;
+            total += (y.n + vb) + qlen[j] + (qlen[j] + nw) + 3 * (nw + 1) + qlen[j] + 1;
+        }
+        keep.emplace_back(total);
+        DBuf fl(4 * nj);
+        if (!keep.back().d || !fl.d) return set_error(PG_ENOMEM, "decimal conversion");
+        uint64_t *A = keep.back().d, *F = fl.d;
+        std::vector<pg_job> j1(nj), j2(nj);
+        std::vector<CarryNode> lq(nj), lr(nj), lt(nj), lc(2 * nj), lv(nj);
+        for (size_t j = 0; j < nj; ++j) {
+            const DNode &y = nodes[jobs_of[j]];
+            uint64_t *ax = A + off[j], *q = ax + y.n + vb, *p = q + qlen[j], *r = p + qlen[j] + nw;
+            uint64_t *t = r + nw + 1, *r2 = t + nw + 1, *q2 = r2 + nw + 1;
+            j1[j] = pg_job{};
+            j1[j].a = y.p; j1[j].an = y.n; j1[j].b = V[i].d; j1[j].bn = vb; j1[j].r = ax; j1[j].r_cap = y.n + vb;
+            lq[j] = lin_node(q, qlen[j]);
+            lq[j].x = ax; lq[j].xn = y.n + vb; lq[j].xsh = (int64_t)sb;
+            j2[j] = pg_job{};
+            j2[j].a = q; j2[j].an = qlen[j]; j2[j].b = W[i].d; j2[j].bn = nw; j2[j].r = p; j2[j].r_cap = qlen[j] + nw;
+            // r = y - q0 W >= 0 (flag 1), fits nw + 1 limbs
+            lr[j] = lin_node(r, std::max(y.n, qlen[j] + nw) + 1);
+            lr[j].out_len = nw + 1;
+            lr[j].x = y.p; lr[j].xn = y.n; lr[j].y = p; lr[j].yn = qlen[j] + nw; lr[j].yneg = 1; lr[j].cin = 1;
+            lr[j].flag = F + 4 * j;
+            // t = r - W, flag f = (r >= W)
+            lt[j] = lin_node(t, nw + 1);
+            lt[j].x = r; lt[j].xn = nw + 1; lt[j].y = W[i].d; lt[j].yn = nw; lt[j].yneg = 1; lt[j].cin = 1;
+            lt[j].flag = F + 4 * j + 1;
+            // r2 = f ? r - W : r,  q2 = q + f
+            lc[2 * j] = lin_node(r2, nw + 1);
+            lc[2 * j].x = r; lc[2 * j].xn = nw + 1; lc[2 * j].y = W[i].d; lc[2 * j].yn = nw; lc[2 * j].yneg = 1;
+            lc[2 * j].cin = 1; lc[2 * j].cond = F + 4 * j + 1; lc[2 * j].cond_eq = 1;
+            lc[2 * j + 1] = lin_node(q2, qlen[j]);
+            lc[2 * j + 1].x = q; lc[2 * j + 1].xn = qlen[j]; lc[2 * j + 1].cin = 1;
+            lc[2 * j + 1].cond = F + 4 * j + 1; lc[2 * j + 1].cond_eq = 1;
+            // check: r2 < W (flag 0)
+            lv[j] = lin_node(t, nw + 1);
+            lv[j].x = r2; lv[j].xn = nw + 1; lv[j].y = W[i].d; lv[j].yn = nw; lv[j].yneg = 1; lv[j].cin = 1;
+            lv[j].flag = F + 4 * j + 2;
+            // children: r < W_i and q = floor(y / W_i) < W_i, q < 2^(bits - wb + 2)
+            const size_t qb = std::min(y.bits + 2 - wb[i], wb[i]);
+            next.push_back({r2, nw, wb[i], y.chunk0});
+            next.push_back({q2, std::min(qlen[j], limbs_for_bits(qb)), qb, y.chunk0 + ((size_t)1 << i)});
+        }
+        {
+            Batch b;
+            b.use_graph = false;
+            if ((st = b.build(j1.data(), nj)) || (st = b.run())) return st;
+        }
+        if ((st = lin_run(lq.data(), nj, "decimal"))) return st;
+        {
+            Batch b;
+            b.use_graph = false;
+            if ((st = b.build(j2.data(), nj)) || (st = b.run())) return st;
+        }
+        if ((st = lin_run(lr.data(), nj, "decimal")) || (st = lin_run(lt.data(), nj, "decimal")) ||
+            (st = lin_run(lc.data(), 2 * nj, "decimal")) || (st = lin_run(lv.data(), nj, "decimal")))
+            return st;
+        std::vector<uint64_t> f(4 * nj);
+        if ((st = read_words(F, f.size(), f.data()))) return st;
+        for (size_t j = 0; j < nj; ++j)
+            if (f[4 * j] != 1 || f[4 * j + 2] != 0) return set_error(PG_EINTERNAL, "decimal split check");
+        keep.push_back(std::move(fl));
+        nodes.swap(next);
+    }
+    // leaves
+    const size_t kcap = (size_t)1 << top;
+    const uint64_t dcap = 19 * (uint64_t)kcap;
+    std::vector<DecLeaf> leaves;
+    for (const DNode &n : nodes) {
+        if (n.n > (size_t)DEC_LEAF) return set_error(PG_EINTERNAL, "decimal leaf too long");
+        leaves.push_back({n.p, (uint32_t)n.n, n.chunk0});
+    }
+    DBuf dl, chars((dcap + 7) / 8);
+    if (!chars.d) return set_error(PG_ENOMEM, "decimal string");
+    if ((st = upload(dl, leaves))) return st;
+    const uint64_t v = (uint64_t)((~(unsigned __int128)0) / TEN19 - ((unsigned __int128)1 << 64));
+    hipEvent_t e0 = nullptr;
+    C.stage_begin(e0);
+    hipLaunchKernelGGL(k_fill_chars, dim3(2048), dim3(256), 0, C.stream, (char *)chars.d, dcap, '0');
+    hipLaunchKernelGGL(k_dec_leaf, dim3(blocks_for(leaves.size(), 64)), dim3(64), 0, C.stream,
+                       (const DecLeaf *)dl.d, (uint32_t)leaves.size(), v, (char *)chars.d, dcap);
+    C.stage_end("decimal_leaf", e0, (double)dcap);
+    HIP_TRY(hipGetLastError());
+    std::string s(dcap, '0');
+    HIP_TRY(hipMemcpyAsync(&s[0], chars.d, dcap, hipMemcpyDeviceToHost, C.stream));
+    if ((st = C.sync())) return st;
+    const size_t first = s.find_first_not_of('0');
+    out = first == std::string::npos ? std::string() : s.substr(first);
+    return PG_OK;
+}
+
+// ---- the pi program ------------------------------------------------------------------
+
+int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {
+    out.clear();
+    int st;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    const size_t guard = 10, work = digits + guard, terms = work / 14 + 2;   // pi_chudnovsky.cpp:163-165
+    auto t0 = now();
+    std::vector<DBuf> keep;
+    DBuf P, Q, T;
+    size_t lp = 0, lq = 0, lt = 0;
+    if ((st = chud_split(terms, P, lp, Q, lq, T, lt, false, keep))) return st;
+    keep.clear();
+    if ((st = ctx().sync())) return st;
+    auto t1 = now();
+    // s = floor(sqrt(10005 * 10^(2 work)))   (pi_chudnovsky.cpp:109-113, 172-174)
+    DBuf pw, s;
+    size_t pwl = 0, sl = 0;
+    {
+        // pow_u32(10, 2 work) (pi_chudnovsky.cpp:62-71): r = r * b on odd bits, b = b * b
+        size_t e = 2 * work;
+        DBuf r(1), b(1);
+        size_t rl = 1, bl = 1;
+        const uint64_t one = 1, ten = 10;
+        if (!r.d || !b.d || (st = dset(r.d, 1, &one, 1)) || (st = dset(b.d, 1, &ten, 1)))
+            return st ? st : set_error(PG_ENOMEM, "power");
+        while (e) {
+            if (e & 1) {
+                DBuf t(rl + bl);
+                if (!t.d || (st = dmul(t.d, r.d, rl, b.d, bl)) || (st = top_len(t.d, rl + bl, &rl)))
+                    return st ? st : set_error(PG_ENOMEM, "power");
+                r = std::move(t);
+            }
+            e >>= 1;
+            if (e) {
+                DBuf t(2 * bl);
+                if (!t.d || (st = dmul(t.d, b.d, bl, b.d, bl)) || (st = top_len(t.d, 2 * bl, &bl)))
+                    return st ? st : set_error(PG_ENOMEM, "power");
+                b = std::move(t);
+            }
+        }
+        pwl = rl + 1;
+        if (!pw.reset(pwl)) return set_error(PG_ENOMEM, "power");
+        CarryNode c = lin_node(pw.d, pwl);   // mul_u32(10^(2 work), 10005)
+        c.x = r.d; c.xn = rl; c.xmul = 10005;
+        if ((st = lin_one(c, "pi")) || (st = top_len(pw.d, pwl, &pwl))) return st;
+        if ((st = isqrt(s, sl, pw.d, pwl))) return st;
+    }
+    if ((st = ctx().sync())) return st;
+    auto t2 = now();
+    // pi_scaled = floor(floor(Q 426880 s / T) / 10^10) = floor(Q 426880 s / (T 10^10))
+    DBuf num, tq, x;
+    size_t xl = 0;
+    {
+        DBuf q2(lq + 1), t2b(lt + 1);
+        if (!q2.d || !t2b.d) return set_error(PG_ENOMEM, "final division");
+        CarryNode c[2] = {lin_node(q2.d, lq + 1), lin_node(t2b.d, lt + 1)};
+        c[0].x = Q.d; c[0].xn = lq; c[0].xmul = 426880;
+        c[1].x = T.d; c[1].xn = lt; c[1].xmul = 10000000000ull;
+        if ((st = lin_run(c, 2, "pi"))) return st;
+        if (!num.reset(lq + 1 + sl)) return set_error(PG_ENOMEM, "final division");
+        if ((st = dmul(num.d, q2.d, lq + 1, s.d, sl))) return st;
+        size_t nl = 0, tl = 0;
+        if ((st = top_len(num.d, lq + 1 + sl, &nl)) || (st = top_len(t2b.d, lt + 1, &tl))) return st;
+        if ((st = div_mulinv(x, xl, nullptr, nullptr, num.d, nl, t2b.d, tl))) return st;
+    }
+    if ((st = ctx().sync())) return st;
+    auto t3 = now();
+    std::string str;
+    if ((st = to_decimal(x.d, xl, str))) return st;
+    auto t4 = now();
+    if (phase_ms) {
+        phase_ms[0] = ms(t0, t1);
+        phase_ms[1] = ms(t1, t2);
+        phase_ms[2] = ms(t2, t3);
+        phase_ms[3] = ms(t3, t4);
+    }
+    if (str.empty()) str = "0";   // to_dec, pi_chudnovsky.cpp:154
+    if (digits == 0) {
+        out = str;
+        return PG_OK;
+    }
+    if (str.size() <= digits) str.insert(0, digits + 1 - str.size(), '0');   // 190-191
+    str.insert(str.end() - (long long)digits, '.');
+    out.swap(str);
+    return PG_OK;
+}
+
+}  // namespace pg

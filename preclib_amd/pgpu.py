@@ -22,7 +22,8 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_buf_capacity", "pg_buf_upload", "pg_buf_download", "pg_buf_ptr", "pg_mul_dev",
            "pg_mul_batch", "pg_batch_plan", "pg_batch_run", "pg_batch_errors", "pg_batch_free",
            "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read",
-           "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev"]
+           "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev", "pg_sqrt", "pg_to_decimal",
+           "pg_chud_split", "pg_pi"]
 
 
 class Job(ctypes.Structure):
@@ -79,6 +80,11 @@ def load(path=None):
                              ctypes.c_void_p, SZ, ctypes.c_void_p, SZ]
     L.pg_recip.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, SZ]
     L.pg_recip_dev.argtypes = [ctypes.c_void_p, SZ, ctypes.POINTER(SZ), ctypes.c_void_p, SZ, SZ]
+    L.pg_sqrt.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ]
+    L.pg_to_decimal.argtypes = [ctypes.c_char_p, SZ, ctypes.POINTER(SZ), P64, SZ]
+    L.pg_chud_split.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, ctypes.POINTER(SZ), P64, SZ,
+                                ctypes.POINTER(SZ), SZ]
+    L.pg_pi.argtypes = [ctypes.c_char_p, SZ, ctypes.POINTER(SZ), SZ, ctypes.POINTER(ctypes.c_double)]
     L.pg_timing_enable.argtypes = [ctypes.c_int]
     L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
     _lib = L
@@ -218,6 +224,54 @@ def recip(b, bits):
     xl = SZ(0)
     _check(_lib.pg_recip(_ptr(x), len(x), ctypes.byref(xl), _ptr(b), len(b), bits))
     return x[:xl.value].copy()
+
+
+def isqrt(a):
+    """floor(sqrt(a)) on the GPU (precn_sqrt semantics)."""
+    if not _inited:
+        init()
+    a = _trimmed(a)
+    x = np.zeros(len(a) // 2 + 2, np.uint64)
+    xl = SZ(0)
+    _check(_lib.pg_sqrt(_ptr(x), len(x), ctypes.byref(xl), _ptr(a), len(a)))
+    return x[:xl.value].copy()
+
+
+def to_decimal(a):
+    """Decimal digits of a limb array (most significant first; '' for zero)."""
+    if not _inited:
+        init()
+    a = _trimmed(a)
+    cap = int(len(a) * 19.3) + 2
+    buf = ctypes.create_string_buffer(cap)
+    n = SZ(0)
+    _check(_lib.pg_to_decimal(buf, cap, ctypes.byref(n), _ptr(a), len(a)))
+    return buf.raw[:n.value].decode()
+
+
+def chud_split(nterms, need_p=True):
+    """(P, Q, T) of the Chudnovsky series over [0, nterms) (T as magnitude)."""
+    if not _inited:
+        init()
+    cap = max(4 * nterms + 8, 16)
+    p, q, t = (np.zeros(cap, np.uint64) for _ in range(3))
+    pl, ql, tl = SZ(0), SZ(0), SZ(0)
+    _check(_lib.pg_chud_split(_ptr(p) if need_p else None, cap, ctypes.byref(pl), _ptr(q), cap, ctypes.byref(ql),
+                              _ptr(t), cap, ctypes.byref(tl), nterms))
+    return (p[:pl.value].copy() if need_p else None), q[:ql.value].copy(), t[:tl.value].copy()
+
+
+def pi(digits, want_phases=False):
+    """The pi program's digit string for `digits` digits (no trailing newline)."""
+    if not _inited:
+        init()
+    cap = digits + 64
+    buf = ctypes.create_string_buffer(cap)
+    n = SZ(0)
+    ph = (ctypes.c_double * 4)()
+    _check(_lib.pg_pi(buf, cap, ctypes.byref(n), digits, ph))
+    s = buf.raw[:n.value].decode()
+    return (s, list(ph)) if want_phases else s
 
 
 def div_dev(q, r, a, an, b, bn):
