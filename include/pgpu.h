@@ -52,9 +52,12 @@ const char *pg_last_error(void);
  * (src/mul_basic.cpp:259-269) and, with path NTT / FFT, the public
  * precn_t mul_ntt(a, b) (src/mul_ntt.cpp:494-518) and
  * precn_t mul_fft(a, b) (src/mul_fft.cpp:386-394).
- * Requires an, bn >= 1, top limbs nonzero, max(an, bn) > 192 and
- * r_cap >= an + bn.  r may alias a or b.  *r_len receives the trimmed length
- * (an+bn-1 or an+bn).  If fft_max_round is not NULL it receives the largest
+ * A zero-length operand gives the empty product (*r_len = 0), as mul_into
+ * does (mul_basic.cpp:239-242).  Otherwise the top limbs must be nonzero and
+ * r_cap >= an + bn; every size runs on the GPU (sides of at most 192 limbs,
+ * or one side of at most 24 under PG_PATH_AUTO, on the small-product kernel
+ * that replaces mul_basic / mul_karatsuba, mul_basic.cpp:76-185).  r may
+ * alias a or b.  *r_len receives the trimmed length (an+bn-1 or an+bn).  If fft_max_round is not NULL it receives the largest
  * |v - nearbyint(v)| over the FP64 coefficients of every FFT run by the call
  * (0 when none ran), the quantity COUNT_FFTS reports (mul_fft.cpp:359-382).
  */
@@ -159,6 +162,40 @@ int pg_chud_split(uint64_t *p, size_t p_cap, size_t *p_len, uint64_t *q, size_t 
  * newline).  phase_ms, if not NULL, receives 4 wall times in ms:
  * binary_split, sqrt_scale, final_div, to_decimal (the --phases lines). */
 int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms);
+
+/* ---- serving: continuous batching of multiply and division requests ------
+ * No counterpart in the reference (single-threaded library calls); this is
+ * the request loop a serving caller puts in front of operator* and
+ * div_mulinv (src/div_newton.cpp:131).  One worker thread owns the device.
+ * Before every round it admits queued requests while their HBM footprint
+ * fits the budget and a slot is free; a round then runs the next step of
+ * every admitted request in shared launches (all products of the round in
+ * one batched launch), so requests that arrive while others are on the GPU
+ * join them as soon as memory and slots free up.  Results are bit-identical
+ * to pg_mul / pg_div on the same operands. */
+enum { PG_OP_MUL = 0, PG_OP_DIV = 1 };
+typedef struct {
+    int op;                      /* PG_OP_MUL: r = a * b; PG_OP_DIV: r = a / b, r2 = a % b  */
+    const uint64_t *a; size_t an;   /* host operands, valid until the request finishes     */
+    const uint64_t *b; size_t bn;
+    uint64_t *r; size_t r_cap;   /* product (an + bn) or quotient (an - bn + 1 limbs)      */
+    uint64_t *r2; size_t r2_cap; /* remainder (bn limbs) or NULL                            */
+    size_t r_len, r2_len;        /* out: trimmed lengths                                    */
+    int status;                  /* out: 1 while pending, then 0 or a PG_E* code           */
+    double t_submit, t_admit, t_done;   /* out: seconds on pg_server_clock()               */
+} pg_request;
+typedef struct {
+    uint64_t submitted, completed, rounds, products, max_active;
+} pg_server_stats;
+typedef struct pg_server pg_server;
+/* hbm_budget 0 = half the library's device budget; max_active 0 = 256 slots. */
+int pg_server_start(pg_server **out, size_t hbm_budget, size_t max_active);
+int pg_server_submit(pg_server *s, pg_request *req);   /* does not block */
+int pg_server_wait(pg_server *s, pg_request *req);     /* returns req->status */
+int pg_server_drain(pg_server *s);                     /* waits for every submitted request */
+int pg_server_read_stats(pg_server *s, pg_server_stats *st);
+int pg_server_stop(pg_server *s);                      /* finishes queued work, joins */
+double pg_server_clock(void);
 
 /* Waits for all queued device work. */
 int pg_sync(void);

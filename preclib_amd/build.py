@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpgpu.so")
 SOURCES = ["pg_ntt.hip", "pg_fft.hip", "pg_small.hip", "pg_carry.hip", "pg_batch.hip", "pg_runtime.hip", "pg_dint.hip",
-           "pg_pi.hip", "pg_api.hip"]
+           "pg_pi.hip", "pg_serve.hip", "pg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -46,5 +46,25 @@ def build(verbose=False, out=LIB, defines=()):
     return out
 
 
+HOST_LIB = os.path.join(HERE, "libprecn.so")
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_precn")
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_host(verbose=False):
+    """The precn_t layer (host C++17 over libpgpu.so) and its C++ test
+    program (which also links the oracle, as its checker)."""
+    inc = os.path.join(ROOT, "include")
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + inc, os.path.join(HERE, "host", "precn.cpp"),
+          "-L" + HERE, "-lpgpu", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB], verbose)
+    orc = os.path.join(ROOT, "oracle")
+    _run([CXX, "-O2", "-std=c++17", "-I" + inc, "-I" + orc, os.path.join(ROOT, "tests", "cpp", "test_precn.cpp"),
+          "-L" + HERE, "-lprecn", "-lpgpu", "-L" + orc, "-loracle",
+          "-Wl,-rpath," + HERE + ":" + orc + ":$ORIGIN/../../preclib_amd:$ORIGIN/../../oracle", "-o", CPP_TEST],
+         verbose)
+    return HOST_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_host(verbose=True))

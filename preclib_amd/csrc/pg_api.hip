@@ -90,27 +90,39 @@ int pg_mul(uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a, size_t a
            size_t bn, int path, double *fft_max_round) {
     LOCK;
     NEED_INIT;
+    if (!an || !bn) {   // a zero operand gives the empty product (mul_basic.cpp:239-242)
+        if (r_len) *r_len = 0;
+        return PG_OK;
+    }
     if (!r || !a || !b) return set_error(PG_EINVAL, "pg_mul: null pointer");
     int st = check_operands(an, bn, r_cap);
     if (st) return st;
     if (!a[an - 1] || !b[bn - 1]) return set_error(PG_EINVAL, "pg_mul: operands must be trimmed");
     Context &C = ctx();
+    // a square (same host operand) is uploaded once, so the device sees
+    // precn_sqr's aliasing (mul_basic.cpp:244-246)
+    const bool same = a == b && an == bn;
     uint64_t *da = (uint64_t *)C.pool.get(an * 8);
-    uint64_t *db = (uint64_t *)C.pool.get(bn * 8);
+    uint64_t *db = same ? da : (uint64_t *)C.pool.get(bn * 8);
     uint64_t *dr = (uint64_t *)C.pool.get((an + bn) * 8);
+    auto release = [&] {
+        C.pool.put(da);
+        if (!same) C.pool.put(db);
+        C.pool.put(dr);
+    };
     if (!da || !db || !dr) {
-        C.pool.put(da); C.pool.put(db); C.pool.put(dr);
+        release();
         return set_error(PG_ENOMEM, "pg_mul: device budget exhausted");
     }
     st = PG_OK;
     if (hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess ||
-        hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess)
+        (!same && hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, C.stream) != hipSuccess))
         st = set_error(PG_EDEVICE, "pg_mul: upload failed");
     if (!st) st = mul_device(dr, da, an, a[an - 1], db, bn, b[bn - 1], path, fft_max_round);
     if (!st && hipMemcpyAsync(r, dr, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess)
         st = set_error(PG_EDEVICE, "pg_mul: download failed");
     if (hipStreamSynchronize(C.stream) != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul: device error");
-    C.pool.put(da); C.pool.put(db); C.pool.put(dr);
+    release();
     if (st) return st;
     size_t len = an + bn;
     while (len && r[len - 1] == 0) --len;
