@@ -1,6 +1,7 @@
 // pg_dint.h -- device-resident integers: the linear steps between products
 // and Newton division (src/div_newton.cpp) on HBM operands.
 #pragma once
+#include <vector>
 #include "pg_runtime.h"
 
 namespace pg {
@@ -37,6 +38,9 @@ int dmul(uint64_t *r, const uint64_t *a, size_t an, const uint64_t *b, size_t bn
 // r[0, n) = value (value limbs, zero padded).
 int dset(uint64_t *r, size_t n, const uint64_t *value, size_t vn);
 int dcopy(uint64_t *r, const uint64_t *a, size_t n);
+
+// floor(2^e / d) for a nonzero host integer d of a few limbs.
+std::vector<uint64_t> pow2_div_small(size_t e, const std::vector<uint64_t> &d);
 
 // Waiting reads.
 int top_len(const uint64_t *a, size_t n, size_t *len);   // trimmed length

@@ -180,7 +180,7 @@ static int cmp_small(const std::vector<uint64_t> &a, const std::vector<uint64_t>
 
 // floor(2^e / d), d nonzero: restoring long division one bit at a time
 // (the role of div_schoolbook(newton64_pow2(precision), bt), div_newton.cpp:47).
-static std::vector<uint64_t> pow2_div_small(size_t e, const std::vector<uint64_t> &d) {
+std::vector<uint64_t> pow2_div_small(size_t e, const std::vector<uint64_t> &d) {
     std::vector<uint64_t> q(e / 64 + 1, 0), r(d.size() + 1, 0);
     for (size_t i = e + 1; i-- > 0;) {
         uint64_t c = (i == e) ? 1 : 0;

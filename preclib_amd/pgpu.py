@@ -23,7 +23,9 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_mul_batch", "pg_batch_plan", "pg_batch_run", "pg_batch_errors", "pg_batch_free",
            "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read",
            "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev", "pg_sqrt", "pg_to_decimal",
-           "pg_chud_split", "pg_pi"]
+           "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
+           "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock"]
+OP_MUL, OP_DIV = 0, 1
 
 
 class Job(ctypes.Structure):
@@ -31,6 +33,19 @@ class Job(ctypes.Structure):
     _fields_ = [("a", ctypes.c_void_p), ("an", SZ), ("b", ctypes.c_void_p), ("bn", SZ),
                 ("r", ctypes.c_void_p), ("r_cap", SZ), ("r_len", SZ), ("path", ctypes.c_int),
                 ("status", ctypes.c_int), ("fft_max_round", ctypes.c_double)]
+
+
+class Request(ctypes.Structure):
+    """pg_request (include/pgpu.h)."""
+    _fields_ = [("op", ctypes.c_int), ("a", ctypes.c_void_p), ("an", SZ), ("b", ctypes.c_void_p), ("bn", SZ),
+                ("r", ctypes.c_void_p), ("r_cap", SZ), ("r2", ctypes.c_void_p), ("r2_cap", SZ),
+                ("r_len", SZ), ("r2_len", SZ), ("status", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+                ("products", ctypes.c_uint64), ("max_active", ctypes.c_uint64)]
 
 
 class PgError(RuntimeError):
@@ -85,6 +100,13 @@ def load(path=None):
     L.pg_chud_split.argtypes = [P64, SZ, ctypes.POINTER(SZ), P64, SZ, ctypes.POINTER(SZ), P64, SZ,
                                 ctypes.POINTER(SZ), SZ]
     L.pg_pi.argtypes = [ctypes.c_char_p, SZ, ctypes.POINTER(SZ), SZ, ctypes.POINTER(ctypes.c_double)]
+    L.pg_server_start.argtypes = [ctypes.POINTER(ctypes.c_void_p), SZ, SZ]
+    L.pg_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request)]
+    L.pg_server_wait.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request)]
+    L.pg_server_drain.argtypes = [ctypes.c_void_p]
+    L.pg_server_read_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+    L.pg_server_stop.argtypes = [ctypes.c_void_p]
+    L.pg_server_clock.restype = ctypes.c_double
     L.pg_timing_enable.argtypes = [ctypes.c_int]
     L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
     _lib = L
@@ -336,3 +358,67 @@ class DevBatch:
             self.free()
         except Exception:
             pass
+
+
+class ServerRequest:
+    """One submitted request; its arrays stay referenced until it is read."""
+
+    def __init__(self, op, a, b):
+        self.a, self.b = _trimmed(a), _trimmed(b)
+        an, bn = len(self.a), len(self.b)
+        if op == OP_MUL:
+            self.r, self.r2 = np.zeros(max(an + bn, 1), np.uint64), None
+        else:
+            self.r, self.r2 = np.zeros(max(an - bn + 1, 1), np.uint64), np.zeros(max(bn, an, 1), np.uint64)
+        self.req = Request(op, self.a.ctypes.data if an else None, an, self.b.ctypes.data if bn else None, bn,
+                           self.r.ctypes.data, len(self.r),
+                           self.r2.ctypes.data if self.r2 is not None else None,
+                           len(self.r2) if self.r2 is not None else 0, 0, 0, 0, 0.0, 0.0, 0.0)
+
+    def result(self):
+        _check(self.req.status)
+        q = self.r[: self.req.r_len].copy()
+        if self.r2 is None:
+            return q
+        return q, self.r2[: self.req.r2_len].copy()
+
+    @property
+    def latency(self):
+        return self.req.t_done - self.req.t_submit
+
+
+class Server:
+    """The continuous-batching request loop (pg_server_*)."""
+
+    def __init__(self, budget=0, max_active=0):
+        if not _inited:
+            init()
+        h = ctypes.c_void_p()
+        _check(_lib.pg_server_start(ctypes.byref(h), budget, max_active))
+        self.h = h
+
+    def submit(self, op, a, b):
+        r = ServerRequest(op, a, b)
+        _check(_lib.pg_server_submit(self.h, ctypes.byref(r.req)))
+        return r
+
+    def wait(self, r):
+        _lib.pg_server_wait(self.h, ctypes.byref(r.req))
+        return r.result()
+
+    def drain(self):
+        _check(_lib.pg_server_drain(self.h))
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.pg_server_read_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def stop(self):
+        if self.h:
+            _lib.pg_server_stop(self.h)
+            self.h = None
+
+    @staticmethod
+    def clock():
+        return load().pg_server_clock()
