@@ -216,6 +216,143 @@ def bench_batch(args, pgpu, workloads):
     batch.free()
 
 
+def bench_pi(args, pgpu):
+    """The pi program (pi_chudnovsky DIGITS) end to end on the GPU: binary
+    splitting, sqrt, final division, decimal conversion; string checked
+    against the reference program's output digest."""
+    d = args.digits
+    gold = {}
+    gp = os.path.join(ROOT, "tests", "golden", "pi.json")
+    if os.path.exists(gp):
+        gold = {e["digits"]: e for e in json.load(open(gp))["pi"]}
+    for _ in range(args.warmup):
+        pgpu.pi(d)
+    times, phases, s = [], [], ""
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        s, ph = pgpu.pi(d, want_phases=True)
+        times.append(time.perf_counter() - t0)
+        phases.append(ph)
+    check = None
+    if d in gold:
+        check = hashlib.sha256((s + "\n").encode()).hexdigest() == gold[d]["sha256"]
+        if not check:
+            print("bench.py: pi digits differ from the reference program's output", file=sys.stderr)
+            sys.exit(1)
+    best = min(times)
+    ph = phases[times.index(best)]
+    line = {
+        "metric": "pi digits/s (Chudnovsky, decimal string, 1 GPU)", "value": d / best, "unit": "digits/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": best * 1e3,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "u32 modular (NTT) / f64 (FFT), u64 limbs", "data": "deterministic (pi)",
+        "config": {"workload": "pi_chudnovsky %d digits" % d, "digits": d, "bit_exact_vs_reference": check},
+        "phases_ms": {"binary_split": ph[0], "sqrt_scale": ph[1], "final_div": ph[2], "to_decimal": ph[3]},
+        "median_ms": float(np.median(times)) * 1e3,
+    }
+    if not args.no_cpu:
+        import subprocess
+        exe = os.path.join(ROOT, "oracle", "_ref", "pi_chudnovsky")
+        if os.path.exists(exe):
+            sd = min(d, 1000000)
+            core = sorted(os.sched_getaffinity(0))[0]
+            t0 = time.perf_counter()
+            subprocess.run(["taskset", "-c", str(core), exe, str(sd)], check=True, capture_output=True)
+            el = time.perf_counter() - t0
+            line["cpu_baseline"] = {"value": sd / el, "unit": "digits/s", "cores": 1, "kind": "reference",
+                                    "sample": "reference pi program, %d digits, one host core (%.1f s)" % (sd, el)}
+    print(json.dumps(line), flush=True)
+
+
+def serve_requests(count, seed, workloads):
+    """C5 requests: 80% multiply (both sizes log-uniform in [256, 2^21]),
+    20% division (divisor log-uniform in [32768, 2^20], dividend above twice
+    the divisor up to 2^21); uniform limbs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = np.log(256), np.log(1 << 21)
+    for k in range(count):
+        if rng.uniform() < 0.8:
+            an, bn = (int(np.exp(rng.uniform(lo, hi))) for _ in range(2))
+            op = 0
+        else:
+            bn = int(np.exp(rng.uniform(np.log(32768), np.log(1 << 20))))
+            an = int(np.exp(rng.uniform(np.log(2 * bn + 1), hi)))
+            op = 1
+        a, b = workloads.pair(an, bn, "uniform", 5000 + k)
+        out.append((op, a, b))
+    return out
+
+
+def bench_serve(args, pgpu, workloads):
+    """Poisson request stream through the continuous-batching server."""
+    import threading
+    reqs = serve_requests(args.requests, 0x5E12E, workloads)
+    limbs = [len(a) + len(b) for _, a, b in reqs]
+    # saturated throughput: every request offered at once
+    srv = pgpu.Server()
+    warm = [srv.submit(op, a, b) for op, a, b in reqs[:8]]
+    srv.drain()
+    del warm
+    t0 = time.perf_counter()
+    hs = [srv.submit(op, a, b) for op, a, b in reqs]
+    srv.drain()
+    sat_s = time.perf_counter() - t0
+    sat_rate = len(reqs) / sat_s
+    st = srv.stats()
+    srv.stop()
+    # alone vs mix: the first few requests alone through the plain entry points
+    for (op, a, b), h in list(zip(reqs, hs))[:6]:
+        r = h.result()
+        ref = pgpu.mul(a, b) if op == 0 else pgpu.divmod_(a, b)
+        same = np.array_equal(r, ref) if op == 0 else all(np.array_equal(x, y) for x, y in zip(r, ref))
+        if not same:
+            print("bench.py: server result differs from the same request alone", file=sys.stderr)
+            sys.exit(1)
+    levels = []
+    for frac in (0.5, 0.8, 1.1):
+        rate = frac * sat_rate
+        rng = np.random.default_rng(int(frac * 1000))
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        srv = pgpu.Server()
+        hs = [None] * len(reqs)
+
+        def feed():
+            t = time.perf_counter()
+            for i, (op, a, b) in enumerate(reqs):
+                t += gaps[i]
+                dt = t - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                hs[i] = srv.submit(op, a, b)
+        t0 = time.perf_counter()
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        srv.drain()
+        wall = time.perf_counter() - t0
+        lat = np.array([h.latency for h in hs]) * 1e3
+        for h in hs:
+            h.result()
+        levels.append({"offered_frac_of_saturation": frac, "offered_req_s": rate, "achieved_req_s": len(reqs) / wall,
+                       "product_limbs_s": sum(limbs) / wall, "p50_ms": float(np.percentile(lat, 50)),
+                       "p99_ms": float(np.percentile(lat, 99)), "max_active": srv.stats()["max_active"]})
+        srv.stop()
+    line = {
+        "metric": "served product limbs/s (80% mul / 20% Newton division, Poisson arrivals)",
+        "value": sum(limbs) / sat_s, "unit": "product limbs/s", "n_gpus": 1, "steps": len(reqs), "warmup": 8,
+        "ms_per_step": sat_s / len(reqs) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 modular (NTT) / f64 (FFT), u64 limbs", "data": "synthetic (splitmix64 uniform limbs)",
+        "config": {"workload": "serving stream, sizes log-uniform [256, 2^21], divisor >= 32768",
+                   "requests": len(reqs), "divisions": sum(1 for r in reqs if r[0] == 1),
+                   "host_to_host": True},
+        "saturated": {"req_s": sat_rate, "rounds": st["rounds"], "products": st["products"],
+                      "max_active": st["max_active"]},
+        "levels": levels,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,8 +361,11 @@ def main():
     ap.add_argument("--logn", type=int, default=22, help="operand size 2^logn limbs")
     ap.add_argument("--dist", default="uniform")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="single", choices=["single", "batch"],
-                    help="single: one 2^22 x 2^22 product (default); batch: the 4096-product uneven batch")
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "pi", "serve"],
+                    help="single: one 2^22 x 2^22 product (default); batch: the 4096-product uneven batch; "
+                         "pi: the pi program at --digits; serve: Poisson request stream")
+    ap.add_argument("--digits", type=int, default=10000000, help="pi workload: decimal digits")
+    ap.add_argument("--requests", type=int, default=240, help="serve workload: requests per rate level")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -245,6 +385,10 @@ def main():
     pgpu.init(0)
     if args.workload == "batch":
         return bench_batch(args, pgpu, workloads)
+    if args.workload == "pi":
+        return bench_pi(args, pgpu)
+    if args.workload == "serve":
+        return bench_serve(args, pgpu, workloads)
     N = 1 << args.logn
     a, b = workloads.pair(N, N, args.dist)
     n = transform_len(a, b)

@@ -396,10 +396,14 @@ class Server:
         h = ctypes.c_void_p()
         _check(_lib.pg_server_start(ctypes.byref(h), budget, max_active))
         self.h = h
+        # the worker writes into a request's struct and arrays until it
+        # finishes: every submitted request stays referenced until then
+        self._live = []
 
     def submit(self, op, a, b):
         r = ServerRequest(op, a, b)
         _check(_lib.pg_server_submit(self.h, ctypes.byref(r.req)))
+        self._live.append(r)
         return r
 
     def wait(self, r):
@@ -408,6 +412,7 @@ class Server:
 
     def drain(self):
         _check(_lib.pg_server_drain(self.h))
+        self._live = []
 
     def stats(self):
         st = ServerStats()
@@ -416,8 +421,15 @@ class Server:
 
     def stop(self):
         if self.h:
-            _lib.pg_server_stop(self.h)
+            _lib.pg_server_stop(self.h)   # finishes queued work first
             self.h = None
+            self._live = []
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
 
     @staticmethod
     def clock():
