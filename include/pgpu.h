@@ -197,6 +197,20 @@ int pg_server_read_stats(pg_server *s, pg_server_stats *st);
 int pg_server_stop(pg_server *s);                      /* finishes queued work, joins */
 double pg_server_clock(void);
 
+/* ---- plan autotuner ----------------------------------------------------
+ * No counterpart in the reference (its transforms are radix-2 sweeps with no
+ * tunable plan).  Each wide-mode NTT length 2^12 .. 2^26 has NTT_VARIANTS = 4
+ * plans (row / column split and column-tile width, pg_kernels.h); all give
+ * the same product.  pg_ntt_autotune times every variant of 2^logn_lo ..
+ * 2^logn_hi (reps runs each, products compared), keeps the fastest in the
+ * library's table and reports ms[(logn - logn_lo) * 4 + variant] (-1 where
+ * a variant equals the default) and chosen[logn - logn_lo].  The table is
+ * read at pg_init from ntt_plans.txt beside libpgpu.so ("logn variant"
+ * lines) or from the file $PGPU_NTT_PLANS names. */
+int pg_ntt_autotune(int logn_lo, int logn_hi, int reps, double *ms, int *chosen);
+int pg_ntt_plan_get(int *variants, int n);        /* variants[logn], logn < n */
+int pg_ntt_plan_set(const int *variants, int n);
+
 /* Waits for all queued device work. */
 int pg_sync(void);
 

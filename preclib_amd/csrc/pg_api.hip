@@ -397,6 +397,31 @@ int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms) {
     return put_string(s, out, cap, len);
 }
 
+// ---- plan autotuner ------------------------------------------------------------------
+
+int pg_ntt_autotune(int logn_lo, int logn_hi, int reps, double *ms, int *chosen) {
+    LOCK;
+    NEED_INIT;
+    if (!ms || !chosen) return set_error(PG_EINVAL, "pg_ntt_autotune: null");
+    return autotune_ntt(logn_lo, logn_hi, reps, ms, chosen);
+}
+
+int pg_ntt_plan_get(int *variants, int n) {
+    LOCK;
+    NEED_INIT;
+    for (int i = 0; i < n; ++i) variants[i] = i <= NTT_MAX_LOG ? ctx().ntt_variant_tab[i] : 0;
+    return PG_OK;
+}
+
+int pg_ntt_plan_set(const int *variants, int n) {
+    LOCK;
+    NEED_INIT;
+    for (int i = 0; i < n && i <= NTT_MAX_LOG; ++i)
+        if (variants[i] < 0 || variants[i] >= NTT_VARIANTS) return set_error(PG_EINVAL, "pg_ntt_plan_set: variant");
+    for (int i = 0; i < n && i <= NTT_MAX_LOG; ++i) ctx().ntt_variant_tab[i] = i >= NTT_VARIANT_MIN_LOG ? variants[i] : 0;
+    return PG_OK;
+}
+
 int pg_timing_enable(int on) {
     LOCK;
     ctx().timing = on != 0;

@@ -376,6 +376,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
             L.io_bytes = b.io;
             L.logn = b.logn;
             L.mode = b.mode;
+            L.variant = ctx().ntt_variant(b.logn, b.mode);
             int st = fill_ntt_tables(L);
             if (st) return st;
         } else {

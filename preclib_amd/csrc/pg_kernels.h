@@ -26,15 +26,36 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 #define PG_ROW_LOG_MAX 15
 #endif
 constexpr int ROW_LOG_MAX = PG_ROW_LOG_MAX;
-constexpr int plan_l2(int logn) {
+constexpr int plan_l2_default(int logn) {
     return logn <= 11 ? logn : cmin(ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 10));
 }
-constexpr int plan_l1(int logn) { return logn - plan_l2(logn); }
-constexpr int plan_lt(int logn) { return cmin(cmin(6, 14 - plan_l1(logn)), plan_l2(logn)); }
+// Plan variants the autotuner chooses from (wide mode, logn >= 12):
+//   0: the default split and tile;  1: column tiles half as wide;
+//   2: rows one level shorter (columns longer);  3: rows one level longer.
+// A variant that does not fit (column tile above 2^14 words, row above
+// 2^ROW_LOG_MAX points) is the default plan.
+constexpr int NTT_VARIANTS = 4, NTT_VARIANT_MIN_LOG = 12;
+constexpr int plan_l2(int logn, int v = 0) {
+    return logn < NTT_VARIANT_MIN_LOG ? plan_l2_default(logn)
+           : (v == 2 && logn - (plan_l2_default(logn) - 1) <= 12) ? plan_l2_default(logn) - 1
+           : (v == 3 && plan_l2_default(logn) + 1 <= ROW_LOG_MAX && logn - plan_l2_default(logn) >= 2)
+               ? plan_l2_default(logn) + 1
+               : plan_l2_default(logn);
+}
+constexpr int plan_l1(int logn, int v = 0) { return logn - plan_l2(logn, v); }
+constexpr int plan_lt(int logn, int v = 0) {
+    return (v == 1 && logn >= NTT_VARIANT_MIN_LOG && cmin(cmin(6, 14 - plan_l1(logn, v)), plan_l2(logn, v)) > 2)
+               ? cmin(cmin(6, 14 - plan_l1(logn, v)), plan_l2(logn, v)) - 1
+               : cmin(cmin(6, 14 - plan_l1(logn, v)), plan_l2(logn, v));
+}
+// Whether variant v differs from the default at this size.
+constexpr bool plan_distinct(int logn, int v) {
+    return v == 0 || plan_l2(logn, v) != plan_l2(logn) || plan_lt(logn, v) != plan_lt(logn);
+}
 // An in-LDS transform of 2^l points as ceil(l/5) radix passes of near-equal
 // size (largest first).
 // Row workgroup: 512 threads, 1024 for 2^15-point rows (32 kept values each).
-constexpr int row_threads(int logn) { return plan_l2(logn) >= 15 ? 1024 : 512; }
+constexpr int row_threads(int logn, int v = 0) { return plan_l2(logn, v) >= 15 ? 1024 : 512; }
 constexpr int split_np(int l) { return l ? (l + 4) / 5 : 0; }
 constexpr int split_lr(int l, int i) {
     return split_np(l) == 0 ? 0 : (i < split_np(l) ? l / split_np(l) + (i < l % split_np(l) ? 1 : 0) : 0);
@@ -79,6 +100,7 @@ struct NttLaunch {
     int nleaves;
     double io_bytes;            // operand + product limb bytes of all leaves (timing model)
     int logn, lobits, nprimes, mode;
+    int variant;                // plan variant (wide mode), chosen by the autotuner table
     PrimeK pk[3];
     CrtK crt;
     const uint2 *col_tw[3][MAX_PASS], *col_itw[3][MAX_PASS];

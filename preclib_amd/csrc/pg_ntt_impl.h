@@ -1,0 +1,530 @@
+// pg_ntt_impl.h -- four-step NTT product kernels for gfx950 (templates; the
+// default plans are instantiated in pg_ntt.hip, autotuner variants in
+// pg_ntt_v1.hip .. pg_ntt_v3.hip).
+//
+// Product path of preclib's mul_ntt (src/mul_ntt.cpp:494-518): base-2^16
+// digits (331-342), cyclic convolution modulo each prime (349-387), Garner
+// CRT (405-412) and base-2^16 carry (438-461).  The transform is organised
+// for HBM instead of the reference's radix-2 sweeps:
+//
+//   n = N1 * N2, digit index j = j1 * N2 + c  (row j1 < N1, column c < N2)
+//   k_col_fwd : per tile of T columns, an N1-point DFT down every column,
+//               then the four-step twiddle omega_n^(c * k1);      HBM sweep 1
+//   k_row     : per row, N2-point DFTs of both operands, pointwise Montgomery
+//               product, inverse N2-point DFT;                      HBM sweep 2
+//   k_col_inv : per tile, inverse twiddle and 1/n, inverse N1-point DFT for
+//               every prime, CRT, and assembly of 4 coefficients into one
+//               128-bit limb sum S_j = sum_t c_{4j+t} 2^(16t);      HBM sweep 3
+//   carry     : S -> normalized limbs (pg_carry.hip).
+//
+// Inside a workgroup each N-point DFT is a sequence of radix-R register
+// passes over an LDS tile (element (row, col) at lds[row * T + col]); pass i
+// with block size B does, for every column j2 < B/R of every block, an R-point
+// DIF in registers and multiplies output k1 by omega_B^(j2 * k1) (Shoup
+// twiddles from a per-(prime, B) table laid out [k1][j2]).  Outputs end in
+// mixed-radix digit-reversed order; the inverse passes run the same network
+// backwards, so no permutation pass is ever needed.
+//
+// Every size parameter is a template constant: one instantiation per
+// transform length 2^8 .. 2^26 and plan variant V (pg_kernels.h), so strides
+// and offsets fold into immediates.
+#pragma once
+#include "pg_modarith.h"
+#include "pg_kernels.h"
+
+namespace pg {
+
+// LDS element i lives at word i + i/32: the stride-32 accesses of the last
+// radix pass then fall in distinct banks.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 5); }
+
+// ---- in-LDS radix passes --------------------------------------------------
+
+template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+__device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict__ tw, const PrimeK &c) {
+    constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
+    constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
+    constexpr int S = M * T;
+#pragma unroll 1
+    for (int col = threadIdx.x; col < NCOLS; col += NT) {
+        const int cc = col & (T - 1), r = col >> LOGT;
+        const int blk = r >> LOGM, j2 = r & (M - 1);
+        const int base = (((blk << LOGB) + j2) << LOGT) + cc;
+        uint32_t x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
+        dft_dif<R>(x, c);
+        if constexpr (M > 1) {
+            const uint2 *t = tw + j2;
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const uint2 w = t[bitrev<R>(q) * M];
+                x[q] = mul_shoup(x[q], w.x, w.y, c.p);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[lp(base + bitrev<R>(q) * S)] = x[q];
+    }
+}
+
+template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+__device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict__ itw, const PrimeK &c) {
+    constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
+    constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
+    constexpr int S = M * T;
+#pragma unroll 1
+    for (int col = threadIdx.x; col < NCOLS; col += NT) {
+        const int cc = col & (T - 1), r = col >> LOGT;
+        const int blk = r >> LOGM, j2 = r & (M - 1);
+        const int base = (((blk << LOGB) + j2) << LOGT) + cc;
+        uint32_t x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = lds[lp(base + bitrev<R>(q) * S)];
+        if constexpr (M > 1) {
+            const uint2 *t = itw + j2;
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                const uint2 w = t[bitrev<R>(q) * M];
+                x[q] = mul_shoup(x[q], w.x, w.y, c.p);
+            }
+        }
+        dft_dit_inv<R>(x, c);
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[lp(base + q * S)] = x[q];
+    }
+}
+
+// Pass lists: forward runs passes 0..np-1 with shrinking block size; the
+// inverse runs them backwards (post-order recursion).
+template <int LOGN, int LOGT, int NT, int LOGB, int I>
+__device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c) {
+    constexpr int LR = split_lr(LOGN, I);
+    if constexpr (LR > 0) {
+        pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c);
+        __syncthreads();
+        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1>(lds, tw, c);
+    }
+}
+
+template <int LOGN, int LOGT, int NT, int LOGB, int I>
+__device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c) {
+    constexpr int LR = split_lr(LOGN, I);
+    if constexpr (LR > 0) {
+        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1>(lds, itw, c);
+        pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c);
+        __syncthreads();
+    }
+}
+
+template <int LOGN, int LOGT, int NT>
+__device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c) {
+    fwd_from<LOGN, LOGT, NT, LOGN, 0>(lds, tw, c);
+}
+
+template <int LOGN, int LOGT, int NT>
+__device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c) {
+    inv_from<LOGN, LOGT, NT, LOGN, 0>(lds, itw, c);
+}
+
+// Frequency index held at position rho after the forward passes of a 2^LOGN
+// transform (mixed-radix digit reversal).
+template <int LOGN>
+__device__ __forceinline__ uint32_t freq_of_pos(uint32_t rho) {
+    uint32_t k = 0;
+    int w = 0, logB = LOGN;
+#pragma unroll
+    for (int i = 0; i < split_np(LOGN); ++i) {
+        const int lr = split_lr(LOGN, i);
+        logB -= lr;
+        k |= ((rho >> logB) & ((1u << lr) - 1)) << w;
+        w += lr;
+    }
+    return k;
+}
+
+__device__ __forceinline__ uint32_t twiddle_mont(const uint32_t *__restrict__ lo,
+                                                 const uint32_t *__restrict__ hi, uint32_t e,
+                                                 int lobits, const PrimeK &c) {
+    return mont_mul(lo[e & ((1u << lobits) - 1)], hi[e >> lobits], c.p, c.ninv);
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin by dispatch order; mapping
+// workgroup i to tile (i % 8) * (ntiles / 8) + i / 8 keeps neighbouring tiles
+// (which share 128-byte lines of every row) on one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int i, int ntiles) {
+    return (ntiles & 7) ? i : (i & 7) * (ntiles >> 3) + (i >> 3);
+}
+
+// x mod p for any x < 2^32 (Shoup's product by 1 with quotient floor(2^32/p)).
+__device__ __forceinline__ uint32_t reduce32(uint32_t x, uint32_t red, uint32_t p) {
+    const uint32_t r = x - __umulhi(x, red) * p;   // [0, 2p)
+    return min(r, r - p);
+}
+
+// The digits of one limb, as residues mod the transform's prime.
+template <int MODE>
+__device__ __forceinline__ void put_digits(uint32_t *d, uint64_t v, uint32_t red, uint32_t p) {
+    if constexpr (MODE == MODE_REF) {
+        d[0] = (uint32_t)(v & 0xFFFF);
+        d[1] = (uint32_t)((v >> 16) & 0xFFFF);
+        d[2] = (uint32_t)((v >> 32) & 0xFFFF);
+        d[3] = (uint32_t)(v >> 48);
+    } else {
+        d[0] = reduce32((uint32_t)v, red, p);
+        d[1] = reduce32((uint32_t)(v >> 32), red, p);
+    }
+}
+
+// ---- kernel 1: forward column pass ------------------------------------------
+
+template <int LOGN, int MODE, int V>
+__global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int DPL = 64 / mode_digit_bits(MODE);   // digits per limb
+    constexpr int N1 = 1 << L1, T = 1 << LT, LPR = T / DPL;
+    constexpr uint32_t NMASK = (1u << LOGN) - 1;
+    constexpr int NP = mode_primes(MODE);
+    extern __shared__ uint32_t lds[];
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), op = blockIdx.y / NP, pr = blockIdx.y % NP;
+    const PrimeK &c = L.pk[pr];
+    const uint32_t c0 = (uint32_t)tile << LT;
+    const uint64_t *src = op ? D.b : D.a;
+    const size_t nl = op ? D.bn : D.an;
+    // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T)
+    for (int e = threadIdx.x; e < N1 * LPR; e += NTT_THREADS) {
+        const int j1 = e / LPR, m = e % LPR;
+        const size_t li = (((size_t)j1 << L2) + c0 + DPL * m) / DPL;
+        // DPL aligned words never straddle a pad
+        put_digits<MODE>(lds + lp(j1 * T + DPL * m), li < nl ? src[li] : 0, L.crt.red[pr], c.p);
+    }
+    __syncthreads();
+    lds_forward<L1, LT, NTT_THREADS>(lds, L.col_tw[pr], c);
+    uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
+    for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
+        const int rho = e >> LT, cc = e & (T - 1);
+        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1>(rho)) & NMASK;
+        uint32_t x = lds[lp(e)];
+        if (ex) x = mont_mul(x, twiddle_mont(L.glo[pr], L.ghi[pr], ex, L.lobits, c), c.p, c.ninv);
+        out[((size_t)rho << L2) + c0 + cc] = x;
+    }
+}
+
+// ---- kernel 2: row pass (forward A, forward B, pointwise, inverse) -------------
+
+template <int LOGN, int MODE, int V>
+__device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int op, int pr,
+                                         int row) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
+    if constexpr (L1 == 0) {
+        // single-row plan: the row is the digit vector itself
+        constexpr int DPL = 64 / mode_digit_bits(MODE);
+        const uint64_t *src = op ? D.b : D.a;
+        const size_t nl = op ? D.bn : D.an;
+        for (int m = threadIdx.x; m < N2 / DPL; m += ROW_THREADS)
+            put_digits<MODE>(lds + lp(DPL * m), (size_t)m < nl ? src[m] : 0, L.crt.red[pr], L.pk[pr].p);
+    } else {
+        const uint4 *in = (const uint4 *)(D.spec + ((size_t)(op * mode_primes(MODE) + pr) << LOGN) +
+                                          ((size_t)row << L2));
+        for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
+            const uint4 v = in[m];
+            uint32_t *d = lds + lp(4 * m);
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
+    }
+}
+
+template <int LOGN, int MODE, int V>
+__global__ void __launch_bounds__(row_threads(LOGN, V)) k_row(const NttLaunch L) {
+    constexpr int L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
+    constexpr int E = (N2 + ROW_THREADS - 1) / ROW_THREADS;
+    extern __shared__ uint32_t lds[];
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const int row = blockIdx.x, pr = blockIdx.y;
+    const PrimeK &c = L.pk[pr];
+    uint32_t keep[E];
+
+    load_row<LOGN, MODE, V>(lds, L, D, 0, pr, row);
+    __syncthreads();
+    lds_forward<L2, 0, ROW_THREADS>(lds, L.row_tw[pr], c);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = threadIdx.x + k * ROW_THREADS;
+        if (i < N2) keep[k] = lds[lp(i)];
+    }
+    __syncthreads();
+    load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
+    __syncthreads();
+    lds_forward<L2, 0, ROW_THREADS>(lds, L.row_tw[pr], c);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int i = threadIdx.x + k * ROW_THREADS;
+        if (i < N2) lds[lp(i)] = mont_mul(lds[lp(i)], keep[k], c.p, c.ninv);
+    }
+    __syncthreads();
+    lds_inverse<L2, 0, ROW_THREADS>(lds, L.row_itw[pr], c);
+    uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
+    for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
+        const uint32_t *d = lds + lp(4 * m);
+        out[m] = make_uint4(d[0], d[1], d[2], d[3]);
+    }
+}
+
+// ---- kernel 3: inverse column pass, CRT, limb sums ------------------------------
+
+// 128-bit accumulate: (lo, hi) += v * 2^sh for a 32-bit v and 0 <= sh < 128.
+__device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t v, int sh) {
+    uint64_t plo, phi;
+    if (sh == 0) { plo = v; phi = 0; }
+    else if (sh < 64) { plo = v << sh; phi = v >> (64 - sh); }
+    else { plo = 0; phi = v << (sh - 64); }
+    const uint64_t s = lo + plo;
+    hi += phi + (s < lo);
+    lo = s;
+}
+
+// Inverse twiddle, 1/n and inverse column transform of prime pr's residues
+// for the tile; result left in LDS.
+template <int LOGN, int V>
+__device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int pr,
+                                           uint32_t c0) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int T = 1 << LT, NE = (1 << L1) * T;
+    constexpr uint32_t NMASK = (1u << LOGN) - 1;
+    const PrimeK &c = L.pk[pr];
+    const uint32_t *in = D.prod + ((size_t)pr << LOGN);
+    for (int e = threadIdx.x; e < NE; e += COLINV_THREADS) {
+        const int rho = e >> LT, cc = e & (T - 1);
+        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1>(rho)) & NMASK;   // ilo/ihi hold omega^-e
+        const uint32_t x = in[((size_t)rho << L2) + c0 + cc];
+        lds[lp(e)] = mont_mul(x, twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
+    }
+    __syncthreads();
+    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS>(lds, L.col_itw[pr], c);
+}
+
+template <int LOGN, int MODE, int V>
+__global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int DPL = 64 / mode_digit_bits(MODE);
+    constexpr int N1 = 1 << L1, T = 1 << LT, NE = N1 * T, LPR = T / DPL, NL = N1 * LPR;
+    constexpr int E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
+    constexpr int EL = (NL + COLINV_THREADS - 1) / COLINV_THREADS;
+    extern __shared__ uint32_t lds[];
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t c0 = (uint32_t)tile << LT;
+    const CrtK &K = L.crt;
+    uint64_t slo[EL], shi[EL];
+#pragma unroll
+    for (int k = 0; k < EL; ++k) { slo[k] = 0; shi[k] = 0; }
+
+    if constexpr (MODE == MODE_REF) {
+        // Two primes; coefficient x = r1 + p1 t (< 2^60), 4 coefficients per limb.
+        uint32_t res0[E];
+        uint64_t coef[E];
+        for (int pr = 0; pr < 2; ++pr) {
+            inv_column<LOGN, V>(lds, L, D, pr, c0);
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int i = threadIdx.x + k * COLINV_THREADS;
+                if (i < NE) {
+                    const uint32_t r = lds[lp(i)];
+                    if (pr == 0) {
+                        res0[k] = r;
+                    } else {
+                        // Garner (mul_ntt.cpp:405-412): x = r1 + p1 * ((r2 - r1) * p1^-1 mod p2)
+                        const uint32_t d = mod_sub(r, res0[k], L.pk[1].p);   // r1 < p1 < p2
+                        const uint32_t t = mul_shoup(d, K.inv12, K.inv12q, L.pk[1].p);
+                        coef[k] = (uint64_t)res0[k] + (uint64_t)L.pk[0].p * t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int i = threadIdx.x + k * COLINV_THREADS;
+                if (i < NE) lds[lp(i)] = half ? (uint32_t)(coef[k] >> 32) : (uint32_t)coef[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < EL; ++k) {
+                const int e = threadIdx.x + k * COLINV_THREADS;
+                if (e < NL) {
+                    const int rho = e / LPR, m = e % LPR;
+                    const uint32_t *w = lds + lp(rho * T + 4 * m);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) acc128(slo[k], shi[k], w[t], 16 * t + 32 * half);
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        // Three primes, Garner: t2 = (r2 - r1) p1^-1 mod p2,
+        // t3 = (r3 - (r1 + p1 t2)) (p1 p2)^-1 mod p3,
+        // x = r1 + p1 t2 + p1 p2 t3 < 2^91; 2 coefficients per limb.
+        uint32_t r0[E], t2[E];
+        for (int pr = 0; pr < 3; ++pr) {
+            inv_column<LOGN, V>(lds, L, D, pr, c0);
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int i = threadIdx.x + k * COLINV_THREADS;
+                if (i < NE) {
+                    const uint32_t r = lds[lp(i)];
+                    if (pr == 0) {
+                        r0[k] = r;
+                    } else if (pr == 1) {
+                        const uint32_t p2 = L.pk[1].p;
+                        t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, p2);
+                    } else {
+                        const uint32_t p3 = L.pk[2].p;
+                        // (r1 + p1 t2) mod p3, with r1 < p1 < p3 and t2 < p2 < p3
+                        const uint32_t x12 = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, p3), p3);
+                        lds[lp(i)] = mul_shoup(mod_sub(r, x12, p3), K.inv123, K.inv123q, p3);   // t3
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // LDS now holds t3; bring t3, r1, t2 of columns (2m, 2m+1) to the limb's thread.
+        uint32_t v3[EL][2], v1[EL][2], v2[EL][2];
+#pragma unroll
+        for (int word = 0; word < 3; ++word) {
+            if (word > 0) {
+#pragma unroll
+                for (int k = 0; k < E; ++k) {
+                    const int i = threadIdx.x + k * COLINV_THREADS;
+                    if (i < NE) lds[lp(i)] = word == 1 ? r0[k] : t2[k];
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int k = 0; k < EL; ++k) {
+                const int e = threadIdx.x + k * COLINV_THREADS;
+                if (e < NL) {
+                    const int rho = e / LPR, m = e % LPR;
+                    const uint32_t *w = lds + lp(rho * T + 2 * m);
+                    uint32_t(&dst)[2] = word == 0 ? v3[k] : word == 1 ? v1[k] : v2[k];
+                    dst[0] = w[0];
+                    dst[1] = w[1];
+                }
+            }
+            __syncthreads();
+        }
+        const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
+        const uint64_t p12lo = (uint32_t)p12, p12hi = p12 >> 32;
+#pragma unroll
+        for (int k = 0; k < EL; ++k) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                // x = (r1 + p1 t2) + p12 * t3, as 128 bits
+                const uint64_t x12 = v1[k][j] + p1 * v2[k][j];
+                const uint64_t a = p12lo * v3[k][j];             // < 2^64
+                const uint64_t b = p12hi * v3[k][j];             // < 2^60
+                uint64_t lo = x12 + a;
+                uint64_t hi = lo < x12;
+                const uint64_t bl = b << 32;
+                lo += bl;
+                hi += (lo < bl) + (b >> 32);
+                // add x * 2^(32 j) into the limb sum
+                if (j == 0) {
+                    const uint64_t s0 = slo[k] + lo;
+                    shi[k] += hi + (s0 < lo);
+                    slo[k] = s0;
+                } else {
+                    acc128(slo[k], shi[k], (uint32_t)lo, 32);
+                    acc128(slo[k], shi[k], (uint32_t)(lo >> 32), 64);
+                    acc128(slo[k], shi[k], (uint32_t)hi, 96);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < EL; ++k) {
+        const int e = threadIdx.x + k * COLINV_THREADS;
+        if (e < NL) {
+            const int rho = e / LPR, m = e % LPR;
+            const size_t li = (((size_t)rho << L2) + c0 + DPL * m) / DPL;
+            if (li < D.sum_len) {
+                D.slo[li] = slo[k];
+                D.shi[li] = shi[k];
+            }
+        }
+    }
+}
+
+// ---- host-side launch table ----------------------------------------------------
+
+template <int LOGN, int MODE, int V>
+static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const StageHook &h) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), NP = mode_primes(MODE);
+    const int ntiles = (1 << L2) >> LT;
+    const size_t col_lds = lds_words((size_t)1 << (L1 + LT)) * 4, row_lds = lds_words((size_t)1 << L2) * 4;
+    const int nl = L.leaves ? L.nleaves : 1;
+    // byte-model shares use the reference's P = 2 (see bench.py)
+    const double n = (double)(1u << LOGN) * nl, io = L.io_bytes;
+    if constexpr (L1 > 0) {
+        h.begin(h.ctx, "ntt_col_fwd");
+        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE, V>), dim3(ntiles, 2 * NP, nl), dim3(NTT_THREADS), col_lds, stream, L);
+        h.end(h.ctx, "ntt_col_fwd", io + 16.0 * NP * n);
+    }
+    h.begin(h.ctx, "ntt_row");
+    hipLaunchKernelGGL((k_row<LOGN, MODE, V>), dim3(1 << L1, NP, nl), dim3(row_threads(LOGN, V)), row_lds, stream, L);
+    h.end(h.ctx, "ntt_row", 24.0 * NP * n + (L1 ? 0.0 : io + 16.0 * NP * n));
+    h.begin(h.ctx, "ntt_col_inv");
+    hipLaunchKernelGGL((k_col_inv<LOGN, MODE, V>), dim3(ntiles, 1, nl), dim3(COLINV_THREADS), col_lds, stream, L);
+    h.end(h.ctx, "ntt_col_inv", 8.0 * NP * n);
+    return hipGetLastError();
+}
+
+template <int LOGN, int MODE, int V>
+static hipError_t set_limits() {
+    constexpr int lim = 156 * 1024;   // dynamic LDS limit; leaves room for static LDS
+    hipError_t e = hipSuccess;
+    if (plan_l1(LOGN, V) > 0)
+        e = hipFuncSetAttribute((const void *)k_col_fwd<LOGN, MODE, V>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_row<LOGN, MODE, V>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_col_inv<LOGN, MODE, V>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    return e;
+}
+
+// Launch / LDS-limit functions of one plan variant over the wide-mode sizes
+// that have variants (2^12 .. 2^26 points).
+#define PG_NTT_VARIANT_SIZES(X) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) \
+    X(24) X(25) X(26)
+template <int N, int V>
+static hipError_t launch_variant(const NttLaunch &L, hipStream_t stream, const StageHook &hook) {
+    if constexpr (plan_distinct(N, V)) return launch_size<N, MODE_WIDE, V>(L, stream, hook);
+    else return hipErrorInvalidValue;
+}
+template <int N, int V>
+static hipError_t limits_variant() {
+    if constexpr (plan_distinct(N, V)) return set_limits<N, MODE_WIDE, V>();
+    else return hipSuccess;
+}
+#define PG_NTT_VARIANT_FUNCS(V)                                                                          \
+    hipError_t launch_ntt_wide_v##V(const NttLaunch &L, hipStream_t stream, const StageHook &hook) {    \
+        switch (L.logn) {                                                                                \
+            PG_NTT_VARIANT_SIZES(PG_VCASE)                                                               \
+        default: return hipErrorInvalidValue;                                                            \
+        }                                                                                                \
+    }                                                                                                    \
+    hipError_t ntt_limits_wide_v##V() {                                                                  \
+        hipError_t e = hipSuccess;                                                                       \
+        PG_NTT_VARIANT_SIZES(PG_VLIM)                                                                    \
+        return e;                                                                                        \
+    }
+
+hipError_t launch_ntt_wide_v1(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
+hipError_t launch_ntt_wide_v2(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
+hipError_t launch_ntt_wide_v3(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
+hipError_t ntt_limits_wide_v1();
+hipError_t ntt_limits_wide_v2();
+hipError_t ntt_limits_wide_v3();
+
+}  // namespace pg

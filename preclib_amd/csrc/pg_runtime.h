@@ -86,6 +86,13 @@ public:
     const FftGlobal &fgtw(int logn);
     FftK fk{};
     uint64_t *maxerr = nullptr;   // device word for the FFT rounding distance
+    // NTT plan variant per transform length (wide mode), from the autotuner's
+    // table (ntt_plans.txt next to libpgpu.so, or $PGPU_NTT_PLANS)
+    int ntt_variant_tab[NTT_MAX_LOG + 1] = {};
+    int ntt_variant(int logn, int mode) const {
+        return (mode == MODE_WIDE && logn >= 0 && logn <= NTT_MAX_LOG) ? ntt_variant_tab[logn] : 0;
+    }
+    void load_plan_table();
     static constexpr size_t NWORDS = 4096;
     uint64_t *words = nullptr;    // device flags / small results of linear steps
 
@@ -138,6 +145,12 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
 // rounding distance of this product (waits for the device).
 int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
                    const uint64_t *bp, size_t bn, uint64_t btop, double *max_round);
+// Times every plan variant of each wide-mode transform length 2^lo .. 2^hi
+// (one product of that length per variant, reps timed runs, identical
+// products required) and keeps the fastest in the context's table.  ms gets
+// (hi - lo + 1) * NTT_VARIANTS entries (-1 where a variant equals the default).
+int autotune_ntt(int lo, int hi, int reps, double *ms, int *chosen);
+
 // mul_into's routing (src/mul_basic.cpp:238-276) on device pointers; atop /
 // btop are the top limbs or ~0 when unknown (full top limbs assumed).
 int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,

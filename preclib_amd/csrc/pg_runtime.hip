@@ -1,7 +1,9 @@
 // pg_runtime.hip -- device context, memory pool, plans and twiddle tables of
 // libpgpu.so, plus the NTT product driver.
 #include "pg_runtime.h"
+#include <dlfcn.h>
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include "pgpu.h"
@@ -135,6 +137,7 @@ int Context::init(int dev, size_t budget) {
     HIP_TRY(fft_set_lds_limits());
     HIP_TRY(hipMalloc(&maxerr, 256));
     HIP_TRY(hipMalloc(&words, NWORDS * sizeof(uint64_t)));
+    load_plan_table();
     {
         const long double PI = 3.141592653589793238462643383279502884L;
         for (int k = 0; k < 8; ++k) {
@@ -169,6 +172,31 @@ void Context::shutdown() {
     hipStreamDestroy(stream);
     stream = nullptr;
     device = -1;
+}
+
+// "logn variant" lines; unknown or invalid entries keep the default plan.
+void Context::load_plan_table() {
+    for (int &v : ntt_variant_tab) v = 0;
+    std::string path;
+    if (const char *e = std::getenv("PGPU_NTT_PLANS")) {
+        path = e;
+    } else {
+        Dl_info info{};
+        if (dladdr((const void *)&ctx, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/ntt_plans.txt";
+        }
+    }
+    FILE *f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+    if (!f) return;
+    int logn = 0, v = 0;
+    char line[256];
+    while (std::fgets(line, sizeof line, f))
+        if (line[0] != '#' && std::sscanf(line, "%d %d", &logn, &v) == 2 && logn >= NTT_VARIANT_MIN_LOG &&
+            logn <= NTT_MAX_LOG && v >= 0 && v < NTT_VARIANTS)
+            ntt_variant_tab[logn] = v;
+    std::fclose(f);
 }
 
 int Context::sync() {
@@ -427,7 +455,8 @@ int fill_ntt_tables(NttLaunch &L) {
     const int logn = L.logn, NP = mode_primes(L.mode);
     L.nprimes = NP;
     L.crt = C.crt;
-    const int l1 = plan_l1(logn), l2 = plan_l2(logn);
+    const int v = L.mode == MODE_WIDE ? L.variant : 0;
+    const int l1 = plan_l1(logn, v), l2 = plan_l2(logn, v);
     for (int k = 0; k < NP; ++k) {
         L.pk[k] = C.pk[k];
         int logB = l1;
@@ -519,8 +548,9 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     L.one.a = ap; L.one.b = bp; L.one.an = an; L.one.bn = bn;
     L.io_bytes = (double)(an + bn) * 16.0;
     L.logn = logn; L.mode = mode;
+    L.variant = C.ntt_variant(logn, mode);
     { int st = fill_ntt_tables(L); if (st) return st; }
-    const int l1 = plan_l1(logn);
+    const int l1 = plan_l1(logn, mode == MODE_WIDE ? L.variant : 0);
     if (l1) L.one.spec = s.get<uint32_t>(2 * NP * n);
     L.one.prod = s.get<uint32_t>(NP * n);
     const size_t m = n / DPL;
@@ -583,6 +613,77 @@ int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
         *max_round = e;
     }
     return PG_OK;
+}
+
+// ---- plan autotuner -------------------------------------------------------------------
+
+int autotune_ntt(int lo, int hi, int reps, double *ms, int *chosen) {
+    Context &C = ctx();
+    if (!C.stream) return set_error(PG_ENOINIT, "pg_init not called");
+    if (lo > hi || reps < 1) return set_error(PG_EINVAL, "autotune: bad range");
+    int st = PG_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int logn = lo; logn <= hi && !st; ++logn) {
+        const int row = logn - lo;
+        for (int v = 0; v < NTT_VARIANTS; ++v) ms[row * NTT_VARIANTS + v] = -1.0;
+        chosen[row] = 0;
+        if (logn < NTT_VARIANT_MIN_LOG || logn > NTT_MAX_LOG) continue;
+        // an = bn = 2^(logn-2) full limbs: 2^logn base-2^32 digits in all
+        const size_t an = (size_t)1 << (logn - 2), bn = an;
+        std::vector<uint64_t> h(an + bn);
+        uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)logn;
+        for (auto &w : h) {
+            x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+            w = x | (1ull << 63);
+        }
+        Scratch s;
+        uint64_t *a = s.get<uint64_t>(an + bn), *r0 = s.get<uint64_t>(an + bn), *r = s.get<uint64_t>(an + bn);
+        if (!a || !r0 || !r) { st = set_error(PG_ENOMEM, "autotune buffers"); break; }
+        HIP_TRY(hipMemcpyAsync(a, h.data(), (an + bn) * 8, hipMemcpyHostToDevice, C.stream));
+        double best = 0.0;
+        const int saved = C.ntt_variant_tab[logn];
+        std::vector<uint64_t> ref, got;
+        for (int v = 0; v < NTT_VARIANTS && !st; ++v) {
+            if (!plan_distinct(logn, v)) continue;
+            C.ntt_variant_tab[logn] = v;
+            uint64_t *out = v ? r : r0;
+            st = ntt_mul_device(out, a, an, ~0ull, a + an, bn, ~0ull, MODE_WIDE);   // warm-up, tables
+            if (st == PG_ERANGE) {   // beyond mul_ntt's caps: the length is never used
+                st = PG_OK;
+                break;
+            }
+            if (!st) st = C.sync();
+            if (st) break;
+            HIP_TRY(hipEventRecord(e0, C.stream));
+            for (int k = 0; k < reps && !st; ++k) st = ntt_mul_device(out, a, an, ~0ull, a + an, bn, ~0ull, MODE_WIDE);
+            HIP_TRY(hipEventRecord(e1, C.stream));
+            if (!st) st = C.sync();
+            if (st) break;
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+            ms[row * NTT_VARIANTS + v] = t / reps;
+            if (v) {   // every variant must give the default plan's product
+                got.resize(an + bn);
+                HIP_TRY(hipMemcpyAsync(got.data(), r, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream));
+                if ((st = C.sync())) break;
+                if (got != ref) { st = set_error(PG_EINTERNAL, "autotune: plan variant changed the product"); break; }
+            } else {
+                ref.resize(an + bn);
+                HIP_TRY(hipMemcpyAsync(ref.data(), r0, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream));
+                if ((st = C.sync())) break;
+            }
+            if (v == 0 || t / reps < best) {
+                best = t / reps;
+                chosen[row] = v;
+            }
+        }
+        C.ntt_variant_tab[logn] = st ? saved : chosen[row];
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return st;
 }
 
 }  // namespace pg

@@ -24,7 +24,9 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_sync", "pg_timing_enable", "pg_timing_reset", "pg_timing_read",
            "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev", "pg_sqrt", "pg_to_decimal",
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
-           "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock"]
+           "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
+           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set"]
+NTT_VARIANTS = 4
 OP_MUL, OP_DIV = 0, 1
 
 
@@ -107,6 +109,10 @@ def load(path=None):
     L.pg_server_read_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
     L.pg_server_stop.argtypes = [ctypes.c_void_p]
     L.pg_server_clock.restype = ctypes.c_double
+    L.pg_ntt_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_int)]
+    L.pg_ntt_plan_get.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.pg_ntt_plan_set.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.pg_timing_enable.argtypes = [ctypes.c_int]
     L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
     _lib = L
@@ -246,6 +252,32 @@ def recip(b, bits):
     xl = SZ(0)
     _check(_lib.pg_recip(_ptr(x), len(x), ctypes.byref(xl), _ptr(b), len(b), bits))
     return x[:xl.value].copy()
+
+
+def ntt_autotune(lo=12, hi=25, reps=5):
+    """Times every NTT plan variant of 2^lo .. 2^hi; returns {logn: (chosen, [ms per variant])}."""
+    if not _inited:
+        init()
+    k = hi - lo + 1
+    ms = (ctypes.c_double * (k * NTT_VARIANTS))()
+    ch = (ctypes.c_int * k)()
+    _check(_lib.pg_ntt_autotune(lo, hi, reps, ms, ch))
+    return {lo + i: (ch[i], [ms[i * NTT_VARIANTS + v] for v in range(NTT_VARIANTS)]) for i in range(k)}
+
+
+def ntt_plans():
+    if not _inited:
+        init()
+    t = (ctypes.c_int * 27)()
+    _check(_lib.pg_ntt_plan_get(t, 27))
+    return list(t)
+
+
+def set_ntt_plans(table):
+    if not _inited:
+        init()
+    t = (ctypes.c_int * len(table))(*table)
+    _check(_lib.pg_ntt_plan_set(t, len(table)))
 
 
 def isqrt(a):

@@ -87,3 +87,23 @@ def test_repeat_is_deterministic(pg):
     r1 = pg.mul(a, b)
     r2 = pg.mul(a, b)
     assert np.array_equal(r1, r2)
+
+
+@pytest.mark.parametrize("v", [1, 2, 3])
+def test_plan_variants(pg, oracle, golden, v):
+    # every autotuner plan variant gives the same product as the oracle and GMP
+    import hashlib
+    saved = pg.ntt_plans()
+    try:
+        pg.set_ntt_plans([v] * 27)
+        for k, (an, bn) in enumerate([(1000, 1100), (3000, 3000), (7000, 9000), (30000, 30000), (100000, 70000),
+                                      (200000, 300000)]):
+            a, b = W.pair(an, bn, "uniform", 40 + k)
+            assert np.array_equal(pg.mul(a, b, path=pg.PATH_NTT), oracle.mul_ntt(a, b)), (v, an, bn)
+        for c in golden["workload"]:
+            if c["an"] in (1 << 20, 1 << 22) and c["dist"] in ("uniform", "ones"):
+                a, b = W.pair(c["an"], c["bn"], c["dist"])
+                r = pg.mul(a, b, path=pg.PATH_NTT)
+                assert hashlib.sha256(r.astype("<u8").tobytes()).hexdigest() == c["sha256"], (v, c["an"], c["dist"])
+    finally:
+        pg.set_ntt_plans(saved)
