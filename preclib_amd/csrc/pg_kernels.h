@@ -29,24 +29,29 @@ constexpr int ROW_LOG_MAX = PG_ROW_LOG_MAX;
 constexpr int plan_l2_default(int logn) {
     return logn <= 11 ? logn : cmin(ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 10));
 }
-// Plan variants the autotuner chooses from (wide mode, logn >= 12):
-//   0: the default split and tile;  1: column tiles half as wide;
-//   2: rows one level shorter (columns longer);  3: rows one level longer.
-// A variant that does not fit (column tile above 2^14 words, row above
-// 2^ROW_LOG_MAX points) is the default plan.
-constexpr int NTT_VARIANTS = 4, NTT_VARIANT_MIN_LOG = 12;
+// Plan variants the autotuner chooses from (wide mode, logn >= 12): the
+// default row length and column-tile width shifted by (VAR_DL2, VAR_DLT):
+//   0: default;  1: tiles half as wide;  2: rows one level shorter;
+//   3: rows one level longer;  4: shorter rows, half tiles;  5: longer rows,
+//   half tiles;  6: rows two levels shorter;  7: tiles a quarter as wide.
+// A variant that does not fit (rows outside 2^5 .. 2^ROW_LOG_MAX, columns
+// outside 2^2 .. 2^12, tiles under 4 columns) is the default plan.
+constexpr int NTT_VARIANTS = 8, NTT_VARIANT_MIN_LOG = 12;
+constexpr int VAR_DL2[NTT_VARIANTS] = {0, 0, -1, 1, -1, 1, -2, 0};
+constexpr int VAR_DLT[NTT_VARIANTS] = {0, -1, 0, 0, -1, -1, 0, -2};
+constexpr int lt_for(int l1, int l2) { return cmin(cmin(6, 14 - l1), l2); }
+constexpr bool variant_fits(int logn, int v) {
+    return v > 0 && v < NTT_VARIANTS && logn >= NTT_VARIANT_MIN_LOG &&
+           plan_l2_default(logn) + VAR_DL2[v] >= 5 && plan_l2_default(logn) + VAR_DL2[v] <= ROW_LOG_MAX &&
+           logn - plan_l2_default(logn) - VAR_DL2[v] >= 2 && logn - plan_l2_default(logn) - VAR_DL2[v] <= 12 &&
+           lt_for(logn - plan_l2_default(logn) - VAR_DL2[v], plan_l2_default(logn) + VAR_DL2[v]) + VAR_DLT[v] >= 2;
+}
 constexpr int plan_l2(int logn, int v = 0) {
-    return logn < NTT_VARIANT_MIN_LOG ? plan_l2_default(logn)
-           : (v == 2 && logn - (plan_l2_default(logn) - 1) <= 12) ? plan_l2_default(logn) - 1
-           : (v == 3 && plan_l2_default(logn) + 1 <= ROW_LOG_MAX && logn - plan_l2_default(logn) >= 2)
-               ? plan_l2_default(logn) + 1
-               : plan_l2_default(logn);
+    return variant_fits(logn, v) ? plan_l2_default(logn) + VAR_DL2[v] : plan_l2_default(logn);
 }
 constexpr int plan_l1(int logn, int v = 0) { return logn - plan_l2(logn, v); }
 constexpr int plan_lt(int logn, int v = 0) {
-    return (v == 1 && logn >= NTT_VARIANT_MIN_LOG && cmin(cmin(6, 14 - plan_l1(logn, v)), plan_l2(logn, v)) > 2)
-               ? cmin(cmin(6, 14 - plan_l1(logn, v)), plan_l2(logn, v)) - 1
-               : cmin(cmin(6, 14 - plan_l1(logn, v)), plan_l2(logn, v));
+    return lt_for(plan_l1(logn, v), plan_l2(logn, v)) + (variant_fits(logn, v) ? VAR_DLT[v] : 0);
 }
 // Whether variant v differs from the default at this size.
 constexpr bool plan_distinct(int logn, int v) {

@@ -1,6 +1,6 @@
 // pg_ntt.hip -- NTT launch table: the default plan of every transform length
 // in both digit modes, and the dispatch to the autotuner's plan variants
-// (pg_ntt_v1.hip .. pg_ntt_v3.hip).  Kernels: pg_ntt_impl.h.
+// (pg_ntt_v1.hip .. pg_ntt_v7.hip).  Kernels: pg_ntt_impl.h.
 #include "pg_ntt_impl.h"
 
 namespace pg {
@@ -14,6 +14,10 @@ hipError_t launch_ntt(const NttLaunch &L, hipStream_t stream, const StageHook &h
         case 1: return launch_ntt_wide_v1(L, stream, hook);
         case 2: return launch_ntt_wide_v2(L, stream, hook);
         case 3: return launch_ntt_wide_v3(L, stream, hook);
+        case 4: return launch_ntt_wide_v4(L, stream, hook);
+        case 5: return launch_ntt_wide_v5(L, stream, hook);
+        case 6: return launch_ntt_wide_v6(L, stream, hook);
+        case 7: return launch_ntt_wide_v7(L, stream, hook);
         default: break;
         }
     }
@@ -41,6 +45,10 @@ hipError_t ntt_set_lds_limits() {
     if (e == hipSuccess) e = ntt_limits_wide_v1();
     if (e == hipSuccess) e = ntt_limits_wide_v2();
     if (e == hipSuccess) e = ntt_limits_wide_v3();
+    if (e == hipSuccess) e = ntt_limits_wide_v4();
+    if (e == hipSuccess) e = ntt_limits_wide_v5();
+    if (e == hipSuccess) e = ntt_limits_wide_v6();
+    if (e == hipSuccess) e = ntt_limits_wide_v7();
     return e;
 }
 

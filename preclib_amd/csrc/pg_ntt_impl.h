@@ -1,6 +1,6 @@
 // pg_ntt_impl.h -- four-step NTT product kernels for gfx950 (templates; the
 // default plans are instantiated in pg_ntt.hip, autotuner variants in
-// pg_ntt_v1.hip .. pg_ntt_v3.hip).
+// pg_ntt_v1.hip .. pg_ntt_v7.hip).
 //
 // Product path of preclib's mul_ntt (src/mul_ntt.cpp:494-518): base-2^16
 // digits (331-342), cyclic convolution modulo each prime (349-387), Garner
@@ -520,11 +520,10 @@ static hipError_t limits_variant() {
         return e;                                                                                        \
     }
 
-hipError_t launch_ntt_wide_v1(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
-hipError_t launch_ntt_wide_v2(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
-hipError_t launch_ntt_wide_v3(const NttLaunch &L, hipStream_t stream, const StageHook &hook);
-hipError_t ntt_limits_wide_v1();
-hipError_t ntt_limits_wide_v2();
-hipError_t ntt_limits_wide_v3();
+#define PG_NTT_VARIANT_DECL(V)                                                                          \
+    hipError_t launch_ntt_wide_v##V(const NttLaunch &L, hipStream_t stream, const StageHook &hook);     \
+    hipError_t ntt_limits_wide_v##V();
+PG_NTT_VARIANT_DECL(1) PG_NTT_VARIANT_DECL(2) PG_NTT_VARIANT_DECL(3) PG_NTT_VARIANT_DECL(4)
+PG_NTT_VARIANT_DECL(5) PG_NTT_VARIANT_DECL(6) PG_NTT_VARIANT_DECL(7)
 
 }  // namespace pg
