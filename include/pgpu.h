@@ -199,11 +199,11 @@ double pg_server_clock(void);
 
 /* ---- plan autotuner ----------------------------------------------------
  * No counterpart in the reference (its transforms are radix-2 sweeps with no
- * tunable plan).  Each wide-mode NTT length 2^12 .. 2^26 has NTT_VARIANTS = 8
+ * tunable plan).  Each wide-mode NTT length 2^12 .. 2^26 has NTT_VARIANTS = 16
  * plans (row / column split and column-tile width, pg_kernels.h); all give
  * the same product.  pg_ntt_autotune times every variant of 2^logn_lo ..
  * 2^logn_hi (reps runs each, products compared), keeps the fastest in the
- * library's table and reports ms[(logn - logn_lo) * 8 + variant] (-1 where
+ * library's table and reports ms[(logn - logn_lo) * 16 + variant] (-1 where
  * a variant equals the default) and chosen[logn - logn_lo].  The table is
  * read at pg_init from ntt_plans.txt beside libpgpu.so ("logn variant"
  * lines) or from the file $PGPU_NTT_PLANS names. */

@@ -11,7 +11,7 @@ constexpr int NTT_THREADS = 1024;           // workgroup size of the column kern
 constexpr int COLINV_THREADS = 512;         // workgroup size of the inverse column kernel
 constexpr int ROW_MAX_ELEMS = 1 << 15;      // longest in-LDS row transform (N2)
 constexpr int COL_MAX_ELEMS = 1 << 14;      // largest column tile (N1 * T)
-constexpr int MAX_PASS = 4;
+constexpr int MAX_PASS = 6;
 
 // LDS words for n elements with one pad word per 32 (see lp() in pg_ntt.hip).
 constexpr size_t lds_words(size_t n) { return n + (n >> 5); }
@@ -29,42 +29,53 @@ constexpr int ROW_LOG_MAX = PG_ROW_LOG_MAX;
 constexpr int plan_l2_default(int logn) {
     return logn <= 11 ? logn : cmin(ROW_LOG_MAX, cmax((logn + 1) / 2, logn - 10));
 }
-// Plan variants the autotuner chooses from (wide mode, logn >= 12): the
-// default row length and column-tile width shifted by (VAR_DL2, VAR_DLT):
+#ifndef PG_RADIX_BITS
+#define PG_RADIX_BITS 4
+#endif
+constexpr int RADIX_BITS = PG_RADIX_BITS;   // largest register pass of the default plans: radix 16
+// An in-LDS transform of 2^l points as ceil(l / rb) radix passes of near-equal
+// size (largest first).
+constexpr int split_np(int l, int rb = RADIX_BITS) { return l ? (l + rb - 1) / rb : 0; }
+constexpr int split_lr(int l, int i, int rb = RADIX_BITS) {
+    return split_np(l, rb) == 0 ? 0
+           : (i < split_np(l, rb) ? l / split_np(l, rb) + (i < l % split_np(l, rb) ? 1 : 0) : 0);
+}
+
+// Plan variants the autotuner chooses from (wide mode, logn >= 12).  Variant
+// v = geometry (v mod 8) + 8 * radix choice.  Geometry g shifts the default
+// row length and column-tile width by (VAR_DL2[g], VAR_DLT[g]):
 //   0: default;  1: tiles half as wide;  2: rows one level shorter;
 //   3: rows one level longer;  4: shorter rows, half tiles;  5: longer rows,
 //   half tiles;  6: rows two levels shorter;  7: tiles a quarter as wide.
-// A variant that does not fit (rows outside 2^5 .. 2^ROW_LOG_MAX, columns
-// outside 2^2 .. 2^12, tiles under 4 columns) is the default plan.
-constexpr int NTT_VARIANTS = 8, NTT_VARIANT_MIN_LOG = 12;
-constexpr int VAR_DL2[NTT_VARIANTS] = {0, 0, -1, 1, -1, 1, -2, 0};
-constexpr int VAR_DLT[NTT_VARIANTS] = {0, -1, 0, 0, -1, -1, 0, -2};
+// Radix choice: passes of at most radix 2^RADIX_BITS (v < 8) or radix 8.
+// A geometry that does not fit (rows outside 2^5 .. 2^ROW_LOG_MAX, columns
+// outside 2^2 .. 2^12, tiles under 4 columns) keeps the default geometry.
+constexpr int NTT_GEOMS = 8, NTT_VARIANTS = 16, NTT_VARIANT_MIN_LOG = 12;
+constexpr int VAR_DL2[NTT_GEOMS] = {0, 0, -1, 1, -1, 1, -2, 0};
+constexpr int VAR_DLT[NTT_GEOMS] = {0, -1, 0, 0, -1, -1, 0, -2};
 constexpr int lt_for(int l1, int l2) { return cmin(cmin(6, 14 - l1), l2); }
-constexpr bool variant_fits(int logn, int v) {
-    return v > 0 && v < NTT_VARIANTS && logn >= NTT_VARIANT_MIN_LOG &&
-           plan_l2_default(logn) + VAR_DL2[v] >= 5 && plan_l2_default(logn) + VAR_DL2[v] <= ROW_LOG_MAX &&
-           logn - plan_l2_default(logn) - VAR_DL2[v] >= 2 && logn - plan_l2_default(logn) - VAR_DL2[v] <= 12 &&
-           lt_for(logn - plan_l2_default(logn) - VAR_DL2[v], plan_l2_default(logn) + VAR_DL2[v]) + VAR_DLT[v] >= 2;
+constexpr bool geom_fits(int logn, int g) {
+    return g > 0 && g < NTT_GEOMS && logn >= NTT_VARIANT_MIN_LOG &&
+           plan_l2_default(logn) + VAR_DL2[g] >= 5 && plan_l2_default(logn) + VAR_DL2[g] <= ROW_LOG_MAX &&
+           logn - plan_l2_default(logn) - VAR_DL2[g] >= 2 && logn - plan_l2_default(logn) - VAR_DL2[g] <= 12 &&
+           lt_for(logn - plan_l2_default(logn) - VAR_DL2[g], plan_l2_default(logn) + VAR_DL2[g]) + VAR_DLT[g] >= 2;
 }
 constexpr int plan_l2(int logn, int v = 0) {
-    return variant_fits(logn, v) ? plan_l2_default(logn) + VAR_DL2[v] : plan_l2_default(logn);
+    return geom_fits(logn, v % NTT_GEOMS) ? plan_l2_default(logn) + VAR_DL2[v % NTT_GEOMS] : plan_l2_default(logn);
 }
 constexpr int plan_l1(int logn, int v = 0) { return logn - plan_l2(logn, v); }
 constexpr int plan_lt(int logn, int v = 0) {
-    return lt_for(plan_l1(logn, v), plan_l2(logn, v)) + (variant_fits(logn, v) ? VAR_DLT[v] : 0);
+    return lt_for(plan_l1(logn, v), plan_l2(logn, v)) + (geom_fits(logn, v % NTT_GEOMS) ? VAR_DLT[v % NTT_GEOMS] : 0);
 }
+// Radix bits of the variant's register passes.
+constexpr int plan_rb(int logn, int v = 0) { return (v >= NTT_GEOMS && logn >= NTT_VARIANT_MIN_LOG) ? 3 : RADIX_BITS; }
 // Whether variant v differs from the default at this size.
 constexpr bool plan_distinct(int logn, int v) {
-    return v == 0 || plan_l2(logn, v) != plan_l2(logn) || plan_lt(logn, v) != plan_lt(logn);
+    return v == 0 || plan_l2(logn, v) != plan_l2(logn) || plan_lt(logn, v) != plan_lt(logn) ||
+           plan_rb(logn, v) != plan_rb(logn);
 }
-// An in-LDS transform of 2^l points as ceil(l/5) radix passes of near-equal
-// size (largest first).
 // Row workgroup: 512 threads, 1024 for 2^15-point rows (32 kept values each).
 constexpr int row_threads(int logn, int v = 0) { return plan_l2(logn, v) >= 15 ? 1024 : 512; }
-constexpr int split_np(int l) { return l ? (l + 4) / 5 : 0; }
-constexpr int split_lr(int l, int i) {
-    return split_np(l) == 0 ? 0 : (i < split_np(l) ? l / split_np(l) + (i < l % split_np(l) ? 1 : 0) : 0);
-}
 
 // Digit modes of the NTT product.
 //  MODE_REF  : preclib's own decomposition (mul_ntt.cpp:331-342, 509-513):

@@ -1,6 +1,6 @@
 // pg_ntt.hip -- NTT launch table: the default plan of every transform length
 // in both digit modes, and the dispatch to the autotuner's plan variants
-// (pg_ntt_v1.hip .. pg_ntt_v7.hip).  Kernels: pg_ntt_impl.h.
+// (pg_ntt_v1.hip .. pg_ntt_v15.hip).  Kernels: pg_ntt_impl.h.
 #include "pg_ntt_impl.h"
 
 namespace pg {
@@ -18,6 +18,14 @@ hipError_t launch_ntt(const NttLaunch &L, hipStream_t stream, const StageHook &h
         case 5: return launch_ntt_wide_v5(L, stream, hook);
         case 6: return launch_ntt_wide_v6(L, stream, hook);
         case 7: return launch_ntt_wide_v7(L, stream, hook);
+        case 8: return launch_ntt_wide_v8(L, stream, hook);
+        case 9: return launch_ntt_wide_v9(L, stream, hook);
+        case 10: return launch_ntt_wide_v10(L, stream, hook);
+        case 11: return launch_ntt_wide_v11(L, stream, hook);
+        case 12: return launch_ntt_wide_v12(L, stream, hook);
+        case 13: return launch_ntt_wide_v13(L, stream, hook);
+        case 14: return launch_ntt_wide_v14(L, stream, hook);
+        case 15: return launch_ntt_wide_v15(L, stream, hook);
         default: break;
         }
     }
@@ -49,6 +57,14 @@ hipError_t ntt_set_lds_limits() {
     if (e == hipSuccess) e = ntt_limits_wide_v5();
     if (e == hipSuccess) e = ntt_limits_wide_v6();
     if (e == hipSuccess) e = ntt_limits_wide_v7();
+    if (e == hipSuccess) e = ntt_limits_wide_v8();
+    if (e == hipSuccess) e = ntt_limits_wide_v9();
+    if (e == hipSuccess) e = ntt_limits_wide_v10();
+    if (e == hipSuccess) e = ntt_limits_wide_v11();
+    if (e == hipSuccess) e = ntt_limits_wide_v12();
+    if (e == hipSuccess) e = ntt_limits_wide_v13();
+    if (e == hipSuccess) e = ntt_limits_wide_v14();
+    if (e == hipSuccess) e = ntt_limits_wide_v15();
     return e;
 }
 

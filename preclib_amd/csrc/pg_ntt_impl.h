@@ -1,6 +1,6 @@
 // pg_ntt_impl.h -- four-step NTT product kernels for gfx950 (templates; the
 // default plans are instantiated in pg_ntt.hip, autotuner variants in
-// pg_ntt_v1.hip .. pg_ntt_v7.hip).
+// pg_ntt_v1.hip .. pg_ntt_v15.hip).
 //
 // Product path of preclib's mul_ntt (src/mul_ntt.cpp:494-518): base-2^16
 // digits (331-342), cyclic convolution modulo each prime (349-387), Garner
@@ -96,45 +96,45 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
 
 // Pass lists: forward runs passes 0..np-1 with shrinking block size; the
 // inverse runs them backwards (post-order recursion).
-template <int LOGN, int LOGT, int NT, int LOGB, int I>
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB>
 __device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c) {
-    constexpr int LR = split_lr(LOGN, I);
+    constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0) {
         pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c);
         __syncthreads();
-        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1>(lds, tw, c);
+        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, tw, c);
     }
 }
 
-template <int LOGN, int LOGT, int NT, int LOGB, int I>
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB>
 __device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c) {
-    constexpr int LR = split_lr(LOGN, I);
+    constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0) {
-        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1>(lds, itw, c);
+        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, itw, c);
         pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c);
         __syncthreads();
     }
 }
 
-template <int LOGN, int LOGT, int NT>
+template <int LOGN, int LOGT, int NT, int RB>
 __device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c) {
-    fwd_from<LOGN, LOGT, NT, LOGN, 0>(lds, tw, c);
+    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, tw, c);
 }
 
-template <int LOGN, int LOGT, int NT>
+template <int LOGN, int LOGT, int NT, int RB>
 __device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c) {
-    inv_from<LOGN, LOGT, NT, LOGN, 0>(lds, itw, c);
+    inv_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, itw, c);
 }
 
 // Frequency index held at position rho after the forward passes of a 2^LOGN
 // transform (mixed-radix digit reversal).
-template <int LOGN>
+template <int LOGN, int RB>
 __device__ __forceinline__ uint32_t freq_of_pos(uint32_t rho) {
     uint32_t k = 0;
     int w = 0, logB = LOGN;
 #pragma unroll
-    for (int i = 0; i < split_np(LOGN); ++i) {
-        const int lr = split_lr(LOGN, i);
+    for (int i = 0; i < split_np(LOGN, RB); ++i) {
+        const int lr = split_lr(LOGN, i, RB);
         logB -= lr;
         k |= ((rho >> logB) & ((1u << lr) - 1)) << w;
         w += lr;
@@ -199,11 +199,11 @@ __global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
         put_digits<MODE>(lds + lp(j1 * T + DPL * m), li < nl ? src[li] : 0, L.crt.red[pr], c.p);
     }
     __syncthreads();
-    lds_forward<L1, LT, NTT_THREADS>(lds, L.col_tw[pr], c);
+    lds_forward<L1, LT, NTT_THREADS, plan_rb(LOGN, V)>(lds, L.col_tw[pr], c);
     uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
     for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
         const int rho = e >> LT, cc = e & (T - 1);
-        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1>(rho)) & NMASK;
+        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;
         uint32_t x = lds[lp(e)];
         if (ex) x = mont_mul(x, twiddle_mont(L.glo[pr], L.ghi[pr], ex, L.lobits, c), c.p, c.ninv);
         out[((size_t)rho << L2) + c0 + cc] = x;
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(row_threads(LOGN, V)) k_row(const NttLaunch L)
 
     load_row<LOGN, MODE, V>(lds, L, D, 0, pr, row);
     __syncthreads();
-    lds_forward<L2, 0, ROW_THREADS>(lds, L.row_tw[pr], c);
+    lds_forward<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_tw[pr], c);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int i = threadIdx.x + k * ROW_THREADS;
@@ -255,14 +255,14 @@ __global__ void __launch_bounds__(row_threads(LOGN, V)) k_row(const NttLaunch L)
     __syncthreads();
     load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
     __syncthreads();
-    lds_forward<L2, 0, ROW_THREADS>(lds, L.row_tw[pr], c);
+    lds_forward<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_tw[pr], c);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int i = threadIdx.x + k * ROW_THREADS;
         if (i < N2) lds[lp(i)] = mont_mul(lds[lp(i)], keep[k], c.p, c.ninv);
     }
     __syncthreads();
-    lds_inverse<L2, 0, ROW_THREADS>(lds, L.row_itw[pr], c);
+    lds_inverse<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_itw[pr], c);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
         const uint32_t *d = lds + lp(4 * m);
@@ -295,12 +295,12 @@ __device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, co
     const uint32_t *in = D.prod + ((size_t)pr << LOGN);
     for (int e = threadIdx.x; e < NE; e += COLINV_THREADS) {
         const int rho = e >> LT, cc = e & (T - 1);
-        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1>(rho)) & NMASK;   // ilo/ihi hold omega^-e
+        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;   // ilo/ihi hold omega^-e
         const uint32_t x = in[((size_t)rho << L2) + c0 + cc];
         lds[lp(e)] = mont_mul(x, twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
     }
     __syncthreads();
-    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS>(lds, L.col_itw[pr], c);
+    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS, plan_rb(LOGN, V)>(lds, L.col_itw[pr], c);
 }
 
 template <int LOGN, int MODE, int V>
@@ -524,6 +524,8 @@ static hipError_t limits_variant() {
     hipError_t launch_ntt_wide_v##V(const NttLaunch &L, hipStream_t stream, const StageHook &hook);     \
     hipError_t ntt_limits_wide_v##V();
 PG_NTT_VARIANT_DECL(1) PG_NTT_VARIANT_DECL(2) PG_NTT_VARIANT_DECL(3) PG_NTT_VARIANT_DECL(4)
-PG_NTT_VARIANT_DECL(5) PG_NTT_VARIANT_DECL(6) PG_NTT_VARIANT_DECL(7)
+PG_NTT_VARIANT_DECL(5) PG_NTT_VARIANT_DECL(6) PG_NTT_VARIANT_DECL(7) PG_NTT_VARIANT_DECL(8)
+PG_NTT_VARIANT_DECL(9) PG_NTT_VARIANT_DECL(10) PG_NTT_VARIANT_DECL(11) PG_NTT_VARIANT_DECL(12)
+PG_NTT_VARIANT_DECL(13) PG_NTT_VARIANT_DECL(14) PG_NTT_VARIANT_DECL(15)
 
 }  // namespace pg

@@ -1,0 +1,10 @@
+// pg_ntt_v11.hip -- NTT plan variant 11 (see plan_l2 / plan_lt / plan_rb in pg_kernels.h)
+// for the wide digit mode, in its own translation unit so the variants
+// compile in parallel.
+#include "pg_ntt_impl.h"
+
+namespace pg {
+#define PG_VCASE(N) case N: return launch_variant<N, 11>(L, stream, hook);
+#define PG_VLIM(N) if (e == hipSuccess) e = limits_variant<N, 11>();
+PG_NTT_VARIANT_FUNCS(11)
+}  // namespace pg

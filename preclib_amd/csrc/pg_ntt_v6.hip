@@ -1,4 +1,4 @@
-// pg_ntt_v6.hip -- NTT plan variant 6 (see plan_l2 / plan_lt in pg_kernels.h)
+// pg_ntt_v6.hip -- NTT plan variant 6 (see plan_l2 / plan_lt / plan_rb in pg_kernels.h)
 // for the wide digit mode, in its own translation unit so the variants
 // compile in parallel.
 #include "pg_ntt_impl.h"

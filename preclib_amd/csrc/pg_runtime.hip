@@ -456,20 +456,20 @@ int fill_ntt_tables(NttLaunch &L) {
     L.nprimes = NP;
     L.crt = C.crt;
     const int v = L.mode == MODE_WIDE ? L.variant : 0;
-    const int l1 = plan_l1(logn, v), l2 = plan_l2(logn, v);
+    const int l1 = plan_l1(logn, v), l2 = plan_l2(logn, v), rb = plan_rb(logn, v);
     for (int k = 0; k < NP; ++k) {
         L.pk[k] = C.pk[k];
         int logB = l1;
-        for (int i = 0; i < split_np(l1); ++i) {
-            const TwTables &t = C.tw(k, logB, split_lr(l1, i));
+        for (int i = 0; i < split_np(l1, rb); ++i) {
+            const TwTables &t = C.tw(k, logB, split_lr(l1, i, rb));
             L.col_tw[k][i] = t.fwd; L.col_itw[k][i] = t.inv;
-            logB -= split_lr(l1, i);
+            logB -= split_lr(l1, i, rb);
         }
         logB = l2;
-        for (int i = 0; i < split_np(l2); ++i) {
-            const TwTables &t = C.tw(k, logB, split_lr(l2, i));
+        for (int i = 0; i < split_np(l2, rb); ++i) {
+            const TwTables &t = C.tw(k, logB, split_lr(l2, i, rb));
             L.row_tw[k][i] = t.fwd; L.row_itw[k][i] = t.inv;
-            logB -= split_lr(l2, i);
+            logB -= split_lr(l2, i, rb);
         }
         const GlobalTw &g = C.gtw(k, logn);
         if (!g.glo) return set_error(PG_ENOMEM, "twiddle tables");

@@ -26,7 +26,7 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
            "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
            "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set"]
-NTT_VARIANTS = 8
+NTT_VARIANTS = 16
 OP_MUL, OP_DIV = 0, 1
 
 

@@ -89,7 +89,7 @@ def test_repeat_is_deterministic(pg):
     assert np.array_equal(r1, r2)
 
 
-@pytest.mark.parametrize("v", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("v", list(range(1, 16)))
 def test_plan_variants(pg, oracle, golden, v):
     # every autotuner plan variant gives the same product as the oracle and GMP
     import hashlib
