@@ -8,7 +8,10 @@
 namespace pg {
 
 constexpr int NTT_THREADS = 1024;           // workgroup size of the column kernels
-constexpr int COLINV_THREADS = 512;         // workgroup size of the inverse column kernel
+#ifndef PG_COLINV_THREADS
+#define PG_COLINV_THREADS 1024
+#endif
+constexpr int COLINV_THREADS = PG_COLINV_THREADS;   // workgroup size of the inverse column kernel
 constexpr int ROW_MAX_ELEMS = 1 << 15;      // longest in-LDS row transform (N2)
 constexpr int COL_MAX_ELEMS = 1 << 14;      // largest column tile (N1 * T)
 constexpr int MAX_PASS = 6;
