@@ -17,7 +17,11 @@ constexpr int COL_MAX_ELEMS = 1 << 14;      // largest column tile (N1 * T)
 constexpr int MAX_PASS = 6;
 
 // LDS words for n elements with one pad word per 32 (see lp() in pg_ntt.hip).
-constexpr size_t lds_words(size_t n) { return n + (n >> 5); }
+#ifndef PG_LDS_PAD_LOG
+#define PG_LDS_PAD_LOG 5
+#endif
+constexpr int LDS_PAD_LOG = PG_LDS_PAD_LOG;   // one pad word after every 2^LDS_PAD_LOG words
+constexpr size_t lds_words(size_t n) { return n + (n >> LDS_PAD_LOG); }
 
 // ---- transform plan, shared by host and kernels ---------------------------
 // n = 2^logn = N1 * N2: rows of N2 = 2^plan_l2 points are transformed in LDS

@@ -34,9 +34,21 @@
 
 namespace pg {
 
-// LDS element i lives at word i + i/32: the stride-32 accesses of the last
-// radix pass then fall in distinct banks.
-__device__ __forceinline__ int lp(int i) { return i + (i >> 5); }
+// LDS element i lives at word i + i / 2^LDS_PAD_LOG, so the strided accesses
+// of the radix passes fall in distinct banks.
+__device__ __forceinline__ int lp(int i) { return i + (i >> LDS_PAD_LOG); }
+
+// Workgroup barrier for LDS traffic: waits for this wave's LDS operations and
+// the other waves, not for its outstanding global loads.  Every exchange
+// between threads of these kernels goes through LDS, so a prefetched global
+// load can stay in flight across the radix passes.
+__device__ __forceinline__ void lds_sync() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+#else
+    lds_sync();
+#endif
+}
 
 // ---- in-LDS radix passes --------------------------------------------------
 
@@ -101,7 +113,7 @@ __device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, 
     constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0) {
         pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c);
-        __syncthreads();
+        lds_sync();
         fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, tw, c);
     }
 }
@@ -112,7 +124,7 @@ __device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw,
     if constexpr (LR > 0) {
         inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, itw, c);
         pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c);
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -198,7 +210,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
         // DPL aligned words never straddle a pad
         put_digits<MODE>(lds + lp(j1 * T + DPL * m), li < nl ? src[li] : 0, L.crt.red[pr], c.p);
     }
-    __syncthreads();
+    lds_sync();
     lds_forward<L1, LT, NTT_THREADS, plan_rb(LOGN, V)>(lds, L.col_tw[pr], c);
     uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
     for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
@@ -244,24 +256,47 @@ __global__ void __launch_bounds__(row_threads(LOGN, V)) k_row(const NttLaunch L)
     const PrimeK &c = L.pk[pr];
     uint32_t keep[E];
 
+    constexpr int E4 = (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS;
+    constexpr bool PF = plan_l1(LOGN, V) > 0 && E4 <= 4;   // larger rows would spill
     load_row<LOGN, MODE, V>(lds, L, D, 0, pr, row);
-    __syncthreads();
+    // operand B's spectrum row is fetched now and lands during A's transform
+    uint4 pb[PF ? E4 : 1];
+    const uint4 *inb = (const uint4 *)(D.spec + ((size_t)(mode_primes(MODE) + pr) << LOGN) + ((size_t)row << L2));
+    if constexpr (PF) {
+#pragma unroll
+        for (int k = 0; k < E4; ++k) {
+            const int m = threadIdx.x + k * ROW_THREADS;
+            if (m < N2 / 4) pb[k] = inb[m];
+        }
+    }
+    lds_sync();
     lds_forward<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_tw[pr], c);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int i = threadIdx.x + k * ROW_THREADS;
         if (i < N2) keep[k] = lds[lp(i)];
     }
-    __syncthreads();
-    load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
-    __syncthreads();
+    lds_sync();
+    if constexpr (PF) {
+#pragma unroll
+        for (int k = 0; k < E4; ++k) {
+            const int m = threadIdx.x + k * ROW_THREADS;
+            if (m < N2 / 4) {
+                uint32_t *d = lds + lp(4 * m);
+                d[0] = pb[k].x; d[1] = pb[k].y; d[2] = pb[k].z; d[3] = pb[k].w;
+            }
+        }
+    } else {
+        load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
+    }
+    lds_sync();
     lds_forward<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_tw[pr], c);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
         const int i = threadIdx.x + k * ROW_THREADS;
         if (i < N2) lds[lp(i)] = mont_mul(lds[lp(i)], keep[k], c.p, c.ninv);
     }
-    __syncthreads();
+    lds_sync();
     lds_inverse<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_itw[pr], c);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
@@ -283,24 +318,81 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t v, i
     lo = s;
 }
 
-// Inverse twiddle, 1/n and inverse column transform of prime pr's residues
-// for the tile; result left in LDS.
-template <int LOGN, int V>
-__device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int pr,
-                                           uint32_t c0) {
+// 128-bit accumulate: (lo, hi) += a * b, shifted left by 32 bits when up32.
+__device__ __forceinline__ void acc_mul128(uint64_t &lo, uint64_t &hi, uint64_t a, uint32_t b, bool up32) {
+    uint64_t plo = a * b, phi = __umul64hi(a, (uint64_t)b);
+    if (up32) {
+        phi = (phi << 32) | (plo >> 32);
+        plo <<= 32;
+    }
+    const uint64_t s = lo + plo;
+    hi += phi + (s < lo);
+    lo = s;
+}
+
+// Prime pr's residues of the tile into registers: element e = threadIdx.x +
+// k * COLINV_THREADS of the tile (row e >> LT, column e & (T - 1)).
+template <int LOGN, int V, int E>
+__device__ __forceinline__ void load_column(uint32_t (&pf)[E], const NttLeaf &D, int pr, uint32_t c0) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int T = 1 << LT, NE = (1 << L1) * T;
+    const uint32_t *in = D.prod + ((size_t)pr << LOGN);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int e = threadIdx.x + k * COLINV_THREADS;
+        if (e < NE) pf[k] = in[((size_t)(e >> LT) << L2) + c0 + (e & (T - 1))];
+    }
+}
+
+// Inverse twiddle and 1/n of prime pr's prefetched residues into LDS.
+template <int LOGN, int V, int E>
+__device__ __forceinline__ void stage_column(uint32_t *lds, const uint32_t (&pf)[E], const NttLaunch &L, int pr,
+                                             uint32_t c0) {
+    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V);
     constexpr int T = 1 << LT, NE = (1 << L1) * T;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     const PrimeK &c = L.pk[pr];
-    const uint32_t *in = D.prod + ((size_t)pr << LOGN);
-    for (int e = threadIdx.x; e < NE; e += COLINV_THREADS) {
-        const int rho = e >> LT, cc = e & (T - 1);
-        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;   // ilo/ihi hold omega^-e
-        const uint32_t x = in[((size_t)rho << L2) + c0 + cc];
-        lds[lp(e)] = mont_mul(x, twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int e = threadIdx.x + k * COLINV_THREADS;
+        if (e < NE) {
+            const int rho = e >> LT, cc = e & (T - 1);
+            const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;   // ilo/ihi hold omega^-e
+            lds[lp(e)] = mont_mul(pf[k], twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
+        }
     }
-    __syncthreads();
-    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS, plan_rb(LOGN, V)>(lds, L.col_itw[pr], c);
+}
+
+// Inverse column transform of prime pr for the tile (result left in LDS);
+// the residues of prime pr + 1 are fetched into pf meanwhile.
+// Tiles of at most 8 residues per thread keep the next prime's residues in
+// flight during the transform; larger ones load each prime just before use
+// (registers would spill).
+template <int E>
+constexpr bool col_prefetch() { return E <= 8; }
+
+template <int LOGN, int V, int E>
+__device__ __forceinline__ void inv_column(uint32_t *lds, uint32_t (&pf)[E], const NttLaunch &L, const NttLeaf &D,
+                                           int pr, int np, uint32_t c0) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    if constexpr (col_prefetch<E>()) {
+        stage_column<LOGN, V, E>(lds, pf, L, pr, c0);
+    } else {   // load, twiddle and store one residue at a time
+        constexpr int T = 1 << LT, NE = (1 << L1) * T;
+        constexpr uint32_t NMASK = (1u << LOGN) - 1;
+        const PrimeK &c = L.pk[pr];
+        const uint32_t *in = D.prod + ((size_t)pr << LOGN);
+        for (int e = threadIdx.x; e < NE; e += COLINV_THREADS) {
+            const int rho = e >> LT, cc = e & (T - 1);
+            const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;
+            const uint32_t x = in[((size_t)rho << L2) + c0 + cc];
+            lds[lp(e)] = mont_mul(x, twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
+        }
+    }
+    lds_sync();
+    if constexpr (col_prefetch<E>())
+        if (pr + 1 < np) load_column<LOGN, V, E>(pf, D, pr + 1, c0);
+    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS, plan_rb(LOGN, V)>(lds, L.col_itw[pr], L.pk[pr]);
 }
 
 template <int LOGN, int MODE, int V>
@@ -323,8 +415,10 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
         // Two primes; coefficient x = r1 + p1 t (< 2^60), 4 coefficients per limb.
         uint32_t res0[E];
         uint64_t coef[E];
+        uint32_t pf[E];
+        if constexpr (col_prefetch<E>()) load_column<LOGN, V, E>(pf, D, 0, c0);
         for (int pr = 0; pr < 2; ++pr) {
-            inv_column<LOGN, V>(lds, L, D, pr, c0);
+            inv_column<LOGN, V, E>(lds, pf, L, D, pr, 2, c0);
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int i = threadIdx.x + k * COLINV_THREADS;
@@ -340,7 +434,7 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
                     }
                 }
             }
-            __syncthreads();
+            lds_sync();
         }
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
@@ -349,7 +443,7 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
                 const int i = threadIdx.x + k * COLINV_THREADS;
                 if (i < NE) lds[lp(i)] = half ? (uint32_t)(coef[k] >> 32) : (uint32_t)coef[k];
             }
-            __syncthreads();
+            lds_sync();
 #pragma unroll
             for (int k = 0; k < EL; ++k) {
                 const int e = threadIdx.x + k * COLINV_THREADS;
@@ -360,15 +454,17 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
                     for (int t = 0; t < 4; ++t) acc128(slo[k], shi[k], w[t], 16 * t + 32 * half);
                 }
             }
-            __syncthreads();
+            lds_sync();
         }
     } else {
         // Three primes, Garner: t2 = (r2 - r1) p1^-1 mod p2,
         // t3 = (r3 - (r1 + p1 t2)) (p1 p2)^-1 mod p3,
         // x = r1 + p1 t2 + p1 p2 t3 < 2^91; 2 coefficients per limb.
         uint32_t r0[E], t2[E];
+        uint32_t pf[E];
+        if constexpr (col_prefetch<E>()) load_column<LOGN, V, E>(pf, D, 0, c0);
         for (int pr = 0; pr < 3; ++pr) {
-            inv_column<LOGN, V>(lds, L, D, pr, c0);
+            inv_column<LOGN, V, E>(lds, pf, L, D, pr, 3, c0);
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int i = threadIdx.x + k * COLINV_THREADS;
@@ -387,10 +483,13 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
                     }
                 }
             }
-            __syncthreads();
+            lds_sync();
         }
-        // LDS now holds t3; bring t3, r1, t2 of columns (2m, 2m+1) to the limb's thread.
-        uint32_t v3[EL][2], v1[EL][2], v2[EL][2];
+        // LDS now holds t3.  The limb's value is sum_j x_j 2^(32 j) over its
+        // columns (2m, 2m+1) with x = r1 + p1 t2 + p1 p2 t3, linear in the
+        // three words, so each word is added into the 128-bit sum as it
+        // arrives: t3 times p1 p2, then r1, then t2 times p1.
+        const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
 #pragma unroll
         for (int word = 0; word < 3; ++word) {
             if (word > 0) {
@@ -399,47 +498,20 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
                     const int i = threadIdx.x + k * COLINV_THREADS;
                     if (i < NE) lds[lp(i)] = word == 1 ? r0[k] : t2[k];
                 }
-                __syncthreads();
+                lds_sync();
             }
+            const uint64_t mult = word == 0 ? p12 : word == 1 ? 1 : p1;
 #pragma unroll
             for (int k = 0; k < EL; ++k) {
                 const int e = threadIdx.x + k * COLINV_THREADS;
                 if (e < NL) {
                     const int rho = e / LPR, m = e % LPR;
                     const uint32_t *w = lds + lp(rho * T + 2 * m);
-                    uint32_t(&dst)[2] = word == 0 ? v3[k] : word == 1 ? v1[k] : v2[k];
-                    dst[0] = w[0];
-                    dst[1] = w[1];
+                    acc_mul128(slo[k], shi[k], mult, w[0], false);
+                    acc_mul128(slo[k], shi[k], mult, w[1], true);
                 }
             }
-            __syncthreads();
-        }
-        const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
-        const uint64_t p12lo = (uint32_t)p12, p12hi = p12 >> 32;
-#pragma unroll
-        for (int k = 0; k < EL; ++k) {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                // x = (r1 + p1 t2) + p12 * t3, as 128 bits
-                const uint64_t x12 = v1[k][j] + p1 * v2[k][j];
-                const uint64_t a = p12lo * v3[k][j];             // < 2^64
-                const uint64_t b = p12hi * v3[k][j];             // < 2^60
-                uint64_t lo = x12 + a;
-                uint64_t hi = lo < x12;
-                const uint64_t bl = b << 32;
-                lo += bl;
-                hi += (lo < bl) + (b >> 32);
-                // add x * 2^(32 j) into the limb sum
-                if (j == 0) {
-                    const uint64_t s0 = slo[k] + lo;
-                    shi[k] += hi + (s0 < lo);
-                    slo[k] = s0;
-                } else {
-                    acc128(slo[k], shi[k], (uint32_t)lo, 32);
-                    acc128(slo[k], shi[k], (uint32_t)(lo >> 32), 64);
-                    acc128(slo[k], shi[k], (uint32_t)hi, 96);
-                }
-            }
+            lds_sync();
         }
     }
 #pragma unroll
