@@ -434,7 +434,10 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_2p22.json")
     if args.logn == 22 and os.path.exists(tpath):
         st = json.load(open(tpath))["stages"].get(dom)
-        traffic = st["traffic_bytes"] if st else None
+        # only a profile of the plan variant this run used (kernel<24, 1, v>)
+        v = pgpu.ntt_plans()[24]
+        if st and st["kernel"].replace(" ", "").find("<24,1,%d>" % v) >= 0:
+            traffic = st["traffic_bytes"]
     line = {
         "metric": "product limbs/s at 2^22 limbs (plain GPU NTT product, device-resident)",
         "value": value,
