@@ -11,7 +11,8 @@
 // Aligned blocks therefore have T >= 0 from level 1 on, the only
 // subtraction is in the level-1 merge, and every later T is a sum.
 //
-// Square root: the divide-and-refine recursion of src/sqrt.cpp:62-86.
+// Square root: Newton's iteration for 1/sqrt at doubling precision (no
+// divisions), a final product and the unit corrections of src/sqrt.cpp:48-58.
 // Decimal conversion: divide and conquer by W_i = (10^19)^(2^i) as
 // src/base_convert.cpp:56-105, with one exact reciprocal per level shared by
 // every node and a one-step quotient correction, then one thread per block
@@ -411,37 +412,12 @@ std::vector<uint64_t> small_isqrt(std::vector<uint64_t> a) {
     return res;
 }
 
-// sqrt_refine_upper (sqrt.cpp:41-60) from an upper bound x (xl limbs).
-int refine_upper(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
+// Makes an estimate x (xl limbs, within a few units of the root) the floor
+// root of a: r = a - x^2, then the two unit loops of sqrt.cpp:48-58.
+int sqrt_fix(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
     int st;
-    for (int it = 0; it < 64; ++it) {   // y = (x + a / x) >> 1 until y >= x
-        size_t xt = 0;
-        if ((st = top_len(x.d, xl, &xt))) return st;
-        DBuf q;
-        size_t ql = 0;
-        if ((st = div_mulinv(q, ql, nullptr, nullptr, a, an, x.d, xt))) return st;
-        const size_t sm = std::max(xt, ql) + 1;
-        DBuf s(sm), y(sm);
-        if (!s.d || !y.d) return set_error(PG_ENOMEM, "sqrt");
-        CarryNode c = lin_node(s.d, sm);
-        c.x = x.d; c.xn = xt; c.y = q.d; c.yn = ql;
-        if ((st = lin_one(c, "sqrt"))) return st;
-        CarryNode h = lin_node(y.d, sm);
-        h.x = s.d; h.xn = sm; h.xsh = 1;
-        if ((st = lin_one(h, "sqrt"))) return st;
-        DBuf dif(sm + 1);
-        CarryNode d = lin_node(dif.d, sm + 1);
-        d.x = y.d; d.xn = sm; d.y = x.d; d.yn = xt; d.yneg = 1; d.cin = 1; d.flag = ctx().words + 16;
-        if ((st = lin_one(d, "sqrt"))) return st;
-        uint64_t f = 0;
-        if ((st = read_words(ctx().words + 16, 1, &f))) return st;
-        if (f) {   // y >= x: x is the floor root unless the bound was not one
-            xl = xt;
-            break;
-        }
-        x = std::move(y);
-        xl = sm;
-    }
+    if ((st = top_len(x.d, xl, &xl))) return st;
+    if (!xl) xl = 1;
     // r = a - x^2, then the two loops of sqrt.cpp:48-58
     const size_t rm = std::max(an, 2 * xl) + 2;
     DBuf sq(2 * xl), r(rm), r2(rm), x2(xl + 1);
@@ -484,13 +460,93 @@ int refine_upper(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
     return set_error(PG_EINTERNAL, "square root correction did not converge");
 }
 
+// floor(sqrt(a)) for a of 5 or more limbs.  With B = bit_length(a), E = B
+// rounded up to even and m = a / 2^E in [1/4, 1), y = Y / 2^P approximates
+// 1/sqrt(m) from below by Newton's iteration y <- y + y (1 - m y^2) / 2 at
+// doubling precision P: each step squares Y, multiplies by the top Q bits of
+// a rounded up (so the correction term never overshoots; its sign word zeroes
+// it when y is already exact to that rounding), multiplies by Y and shifts.
+// Then s = floor(a Y / 2^(P + E/2)) is the root or one below it, and
+// sqrt_fix makes it exact.  Every multiply runs on the GPU product engine.
+int isqrt_newton(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
+    int st;
+    size_t B = 0;
+    if ((st = bit_length(a, an, &B))) return st;
+    const size_t E = B + (B & 1);
+    // seed: s_t = isqrt of the top 128 bits (Mt = floor(a / 2^(E - 128))),
+    // Y0 = floor(2^(P0 + 64) / (s_t + 1)) <= 2^(P0 + E/2) / sqrt(a)
+    const size_t QT = 128, drop = E - QT, l0 = drop / 64;
+    std::vector<uint64_t> w(an - l0), mt(3, 0);
+    if ((st = read_words(a + l0, w.size(), w.data()))) return st;
+    {
+        const int r = (int)(drop % 64);
+        for (size_t i = 0; i < 3 && i < w.size(); ++i)
+            mt[i] = r ? (w[i] >> r) | (i + 1 < w.size() ? w[i + 1] << (64 - r) : 0) : w[i];
+        mt.resize(2);
+    }
+    std::vector<uint64_t> stt = small_isqrt(mt);
+    stt.push_back(0);
+    for (auto &v : stt)
+        if (++v) break;
+    while (!stt.empty() && !stt.back()) stt.pop_back();
+    size_t P = 60;
+    const std::vector<uint64_t> y0 = pow2_div_small(P + QT / 2, stt);
+    size_t yl = limbs_for_bits(P + 2) + 1;
+    DBuf Y(yl), F(1);
+    if (!Y.d || !F.d || (st = dset(Y.d, yl, y0.data(), y0.size()))) return st ? st : set_error(PG_ENOMEM, "sqrt");
+    const size_t target = B / 2 + 64;
+    while (P < target) {
+        const size_t P2 = std::min(target, 2 * P - 8), Q = P2 + 8;
+        // mt = top Q bits of a, rounded up
+        const size_t mtl = limbs_for_bits(Q + 1);
+        DBuf M(mtl), Y2(2 * yl), T(mtl + 2 * yl);
+        if (!M.d || !Y2.d || !T.d) return set_error(PG_ENOMEM, "sqrt");
+        CarryNode c = lin_node(M.d, mtl);
+        c.x = a; c.xn = an; c.xsh = (int64_t)(E - Q); c.cin = 1;
+        if ((st = lin_one(c, "sqrt"))) return st;
+        if ((st = dmul(Y2.d, Y.d, yl, Y.d, yl)) || (st = dmul(T.d, M.d, mtl, Y2.d, 2 * yl))) return st;
+        // D = 2^(Q + 2P) - mt Y^2 (sign word F), Du = D if D >= 0 else 0
+        const size_t dm = std::max(limbs_for_bits(Q + 2 * P + 1), mtl + 2 * yl) + 1;
+        DBuf D(dm), Du(dm);
+        if (!D.d || !Du.d) return set_error(PG_ENOMEM, "sqrt");
+        CarryNode d = lin_node(D.d, dm);
+        d.pow2 = (int64_t)(Q + 2 * P); d.y = T.d; d.yn = mtl + 2 * yl; d.yneg = 1; d.cin = 1; d.flag = F.d;
+        CarryNode u = lin_node(Du.d, dm);
+        u.y = D.d; u.yn = dm; u.cond = F.d; u.cond_eq = 1;
+        if ((st = lin_one(d, "sqrt")) || (st = lin_one(u, "sqrt"))) return st;
+        size_t dl = 0;
+        if ((st = top_len(Du.d, dm, &dl))) return st;
+        // Y' = Y 2^(P2 - P) + floor(Y Du / 2^(Q + 3P - P2 + 1))
+        const size_t yl2 = limbs_for_bits(P2 + 2) + 1;
+        DBuf C(yl + std::max<size_t>(dl, 1)), Yn(yl2);
+        if (!C.d || !Yn.d) return set_error(PG_ENOMEM, "sqrt");
+        if ((st = dmul(C.d, Y.d, yl, Du.d, std::max<size_t>(dl, 1)))) return st;
+        CarryNode n = lin_node(Yn.d, yl2);
+        n.x = Y.d; n.xn = yl; n.xsh = -(int64_t)(P2 - P);
+        n.y = C.d; n.yn = yl + std::max<size_t>(dl, 1); n.ysh = (int64_t)(Q + 3 * P - P2 + 1);
+        if ((st = lin_one(n, "sqrt"))) return st;
+        Y = std::move(Yn);
+        yl = yl2;
+        P = P2;
+    }
+    // s = floor(a Y / 2^(P + E/2))
+    xl = limbs_for_bits(B / 2 + 2) + 1;
+    DBuf AY(an + yl);
+    if (!AY.d || !x.reset(xl)) return set_error(PG_ENOMEM, "sqrt");
+    if ((st = dmul(AY.d, a, an, Y.d, yl))) return st;
+    CarryNode c = lin_node(x.d, xl);
+    c.x = AY.d; c.xn = an + yl; c.xsh = (int64_t)(P + E / 2);
+    if ((st = lin_one(c, "sqrt"))) return st;
+    return sqrt_fix(x, xl, a, an);
+}
+
 }  // namespace
 
 int isqrt(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
     xl = 0;
     int st;
     if (!an) return PG_OK;
-    if (an <= 4) {   // sqrt.cpp:78: small operands from the top seed
+    if (an <= 4) {   // sqrt.cpp:78: small operands on the host
         std::vector<uint64_t> h(an);
         if ((st = read_words(a, an, h.data()))) return st;
         const std::vector<uint64_t> r = small_isqrt(h);
@@ -498,17 +554,7 @@ int isqrt(DBuf &x, size_t &xl, const uint64_t *a, size_t an) {
         if (!x.reset(std::max<size_t>(xl, 1))) return set_error(PG_ENOMEM, "sqrt");
         return dset(x.d, std::max<size_t>(xl, 1), r.data(), r.size());
     }
-    const size_t drop = (an / 2) & ~(size_t)1;   // sqrt.cpp:80
-    DBuf h;
-    size_t hl = 0;
-    if ((st = isqrt(h, hl, a + drop, an - drop))) return st;
-    // upper = (high_root + 1) << (32 drop)   (sqrt.cpp:84)
-    xl = hl + 1 + drop / 2;
-    if (!x.reset(xl)) return set_error(PG_ENOMEM, "sqrt");
-    CarryNode c = lin_node(x.d, xl);
-    c.x = h.d; c.xn = hl; c.xsh = -(int64_t)(32 * drop); c.pow2 = (int64_t)(32 * drop);   // (h << s) + 2^s
-    if ((st = lin_one(c, "sqrt"))) return st;
-    return refine_upper(x, xl, a, an);
+    return isqrt_newton(x, xl, a, an);
 }
 
 // ---- decimal conversion ------------------------------------------------------------
