@@ -35,8 +35,7 @@ const char *pg_last_error(void) { return pg::last_error(); }
 int pg_sync(void) {
     LOCK;
     NEED_INIT;
-    if (hipStreamSynchronize(ctx().stream) != hipSuccess) return set_error(PG_EDEVICE, "stream sync failed");
-    return PG_OK;
+    return ctx().sync();
 }
 
 int pg_buf_alloc(pg_buf **h, size_t limbs) {
@@ -481,7 +480,8 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
                 st = b.read_errors(e);
                 if (!st) *fft_max_round = e[0];
             }
-            if (!st && hipStreamSynchronize(ctx().stream) != hipSuccess) st = set_error(PG_EDEVICE, "device error");
+            // stream-ordered: the batch's arena returns to the pool, whose
+            // blocks are only reused by later work on the same stream
             return st;
         }
         path = (!sqr && hi > 6144) ? PG_PATH_NTT : PG_PATH_FFT;

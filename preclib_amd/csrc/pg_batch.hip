@@ -390,10 +390,11 @@ int Batch::build(const pg_job *jobs, size_t n) {
             if (st) return st;
         }
     }
-    // one upload of every descriptor
-    if (hipMemcpyAsync(P(d_ntt), img.data(), img.size(), hipMemcpyHostToDevice, C.stream) != hipSuccess ||
-        hipStreamSynchronize(C.stream) != hipSuccess)
+    // one upload of every descriptor (stream-ordered; the host image stays
+    // alive until the context's next sync)
+    if (hipMemcpyAsync(P(d_ntt), img.data(), img.size(), hipMemcpyHostToDevice, C.stream) != hipSuccess)
         return set_error(PG_EDEVICE, "pg_batch: descriptor upload failed");
+    C.hold(std::move(img));
     return PG_OK;
 }
 
