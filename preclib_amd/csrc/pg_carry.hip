@@ -12,8 +12,9 @@
 // Y = ballot(g), the carry into lane i is bit i of (X + Y + cin) ^ X ^ Y and
 // the carry out of the group is the overflow of that 64-bit sum.  Each wave
 // walks CARRY_ITERS such groups (coalesced loads, the wave carry chained in
-// registers); chunks of CARRY_CHUNK limbs are joined by a per-chunk
-// aggregate and a one-workgroup scan.  Three launches: reduce, scan, apply.
+// registers); chunks of CARRY_CHUNK limbs take their carry-in from the limbs
+// just below them and are checked and, where needed, fixed up by a
+// one-workgroup scan of the chunk aggregates (see k_carry_spec).
 // Linear nodes reuse the same scan for device additions, subtractions,
 // shifts and small-scalar multiples (the add/sub/shift/mul_u32 steps that
 // div_newton.cpp, sqrt.cpp and pi_chudnovsky.cpp run between products).
@@ -133,28 +134,6 @@ __device__ __forceinline__ Agg compose(Agg lo, Agg hi) {   // lo first, then hi
     return Agg{hi.g | (hi.p & lo.g), hi.p & lo.p};
 }
 
-__global__ void __launch_bounds__(CARRY_THREADS) k_carry_reduce(const CarryLaunch C) {
-    __shared__ Agg waves[CARRY_WAVES];
-    const CarryNode &N = node_of(C, blockIdx.x);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t base = (blockIdx.x - N.chunk0) * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
-    Agg a{0, 1};
-#pragma unroll
-    for (int it = 0; it < CARRY_ITERS; ++it) {
-        uint32_t g, p;
-        limb_t(N, base + 64 * it, g, p);
-        const Group G{__ballot(g | p), __ballot(g)};
-        a = compose(a, group_agg(G));
-    }
-    if (lane == 0) waves[wave] = a;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Agg b{0, 1};
-        for (int i = 0; i < CARRY_WAVES; ++i) b = compose(b, waves[i]);
-        C.agg[blockIdx.x] = (uint8_t)(b.g | (b.p << 1));
-    }
-}
-
 // One workgroup: exclusive scan of chunk aggregates into chunk carry-ins.
 // The first chunk of every node starts with carry 0 (a "kill" before it).
 __device__ __forceinline__ bool node_start(const CarryLaunch &C, size_t i) {
@@ -175,7 +154,7 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
         const uint32_t v = C.agg[i];
         if (node_start(C, i)) a = Agg{node_cin(C, i), 0};
-        a = compose(a, Agg{v & 1, v >> 1});
+        a = compose(a, Agg{v & 1, (v >> 1) & 1});
     }
     // thread-level (G, P) pairs -> carries via the wave sum trick
     const Group G{__ballot(a.g | a.p), __ballot(a.g)};
@@ -199,29 +178,139 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
     }
 }
 
-__global__ void __launch_bounds__(CARRY_THREADS) k_carry_apply(const CarryLaunch C) {
+// ---- speculative form: carry-in from the limbs just below each chunk --------
+//
+// The carry into a chunk is the carry out of the 64 limbs below it unless all
+// 64 propagate: then the nearest limb below that generates or kills decides.
+// k_carry_spec computes that window with the chunk itself and applies the
+// carry it gives, so every chunk is written in one streaming pass with no
+// dependence on other workgroups.  A chunk whose window propagates
+// throughout is written with carry-in 0 and marked unresolved; the scan then
+// gives every chunk its true carry-in from the chunk aggregates, and
+// k_carry_fix adds the missing 1 to the unresolved chunks whose carry-in is
+// 1 (a +1 that ripples through the chunk's propagating prefix).
+//
+// agg[i] = g | p << 1 | used carry-in << 2 | resolved << 3.
+
+// Limbs t_j, j = base + 64 it, of a single plain node with whole-limb sums
+// (the product path).  Every load is issued before the first is used: the
+// indices are clamped into [0, m) and the out-of-range words discarded after.
+template <int IT>
+__device__ __forceinline__ void limbs_t_plain(const CarryNode &N, long base, uint64_t (&t)[IT],
+                                              uint32_t (&g)[IT], uint32_t (&p)[IT]) {
+    const uint64_t *__restrict__ slo = N.slo;
+    const uint64_t *__restrict__ shi = N.shi;
+    const long m = (long)N.m;
+    uint64_t lo[IT], hp[IT];
+    if (m) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const long j = base + 64 * it;
+            const long jl = j < 0 ? 0 : (j < m ? j : m - 1);
+            const long jh = j < 1 ? 0 : (j - 1 < m ? j - 1 : m - 1);
+            lo[it] = slo[jl];
+            hp[it] = shi[jh];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const long j = base + 64 * it;
+        const uint64_t hprev = (j >= 1 && j <= m) ? hp[it] : 0;
+        if (j >= 0 && j < m) {
+            t[it] = lo[it] + hprev;
+            g[it] = t[it] < lo[it];
+            p[it] = t[it] == ~0ull;
+        } else {
+            t[it] = (j >= 0 && j == m) ? hprev : 0;
+            g[it] = 0;
+            p[it] = 0;
+        }
+    }
+}
+
+template <bool PLAIN, int IT>
+__device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint64_t (&t)[IT], uint32_t (&g)[IT],
+                                            uint32_t (&p)[IT]) {
+    if constexpr (PLAIN) {
+        limbs_t_plain<IT>(N, base, t, g, p);
+    } else {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const long j = base + 64 * it;
+            if (j >= 0) {
+                t[it] = limb_t(N, (size_t)j, g[it], p[it]);
+            } else {
+                t[it] = 0;
+                g[it] = 0;
+                p[it] = 0;
+            }
+        }
+    }
+}
+
+// PLAIN: C.one is a plain node with whole-limb sums and there is no node table.
+template <bool PLAIN>
+__global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch C) {
     __shared__ Agg waves[CARRY_WAVES];
     __shared__ uint32_t wave_in[CARRY_WAVES];
-    const CarryNode &N = node_of(C, blockIdx.x);
+    const size_t chunk = blockIdx.x;
+    const CarryNode &N = PLAIN ? C.one : node_of(C, chunk);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t base = (blockIdx.x - N.chunk0) * CARRY_CHUNK + (size_t)wave * (64 * CARRY_ITERS) + lane;
-    uint64_t t[CARRY_ITERS];
+    const long cbase = (long)(chunk - N.chunk0) * CARRY_CHUNK;
+    const long base = cbase + (long)wave * (64 * CARRY_ITERS) + lane;
+    const bool start = PLAIN ? chunk == 0 : node_start(C, chunk);
+    // wave 0 also reads the window: the 64 limbs below the chunk
+    constexpr int IT1 = CARRY_ITERS + 1;
+    uint64_t t[IT1];
+    uint32_t g[IT1], p[IT1];
+    if (wave == 0 && !start) {
+        chunk_limbs<PLAIN, IT1>(N, base - 64, t, g, p);
+    } else {
+        uint64_t t0[CARRY_ITERS];
+        uint32_t g0[CARRY_ITERS], p0[CARRY_ITERS];
+        chunk_limbs<PLAIN, CARRY_ITERS>(N, base, t0, g0, p0);
+        t[0] = 0; g[0] = 0; p[0] = 0;
+#pragma unroll
+        for (int it = 0; it < CARRY_ITERS; ++it) {
+            t[it + 1] = t0[it];
+            g[it + 1] = g0[it];
+            p[it + 1] = p0[it];
+        }
+    }
     Group G[CARRY_ITERS];
     Agg a{0, 1};
 #pragma unroll
     for (int it = 0; it < CARRY_ITERS; ++it) {
-        uint32_t g, p;
-        t[it] = limb_t(N, base + 64 * it, g, p);
-        G[it] = Group{__ballot(g | p), __ballot(g)};
+        G[it] = Group{__ballot(g[it + 1] | p[it + 1]), __ballot(g[it + 1])};
         a = compose(a, group_agg(G[it]));
     }
     if (lane == 0) waves[wave] = a;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t x = C.cin[blockIdx.x];
-        for (int i = 0; i < CARRY_WAVES; ++i) {
-            wave_in[i] = x;
-            x = waves[i].g | (waves[i].p & x);
+    if (threadIdx.x < 64) {
+        uint32_t cin, resolved = 1;
+        if (start) {
+            cin = PLAIN ? 0u : node_cin(C, chunk);
+        } else {
+            // window: carry out of the 64 limbs below, if some limb there does
+            // not propagate (the highest such limb decides)
+            const uint64_t np = __ballot(!p[0]), gb = __ballot(g[0]);
+            if (np) {
+                const int k = 63 - __clzll((long long)np);
+                cin = (uint32_t)((gb >> k) & 1);
+            } else {
+                cin = 0;
+                resolved = 0;
+            }
+        }
+        if (threadIdx.x == 0) {
+            Agg b{0, 1};
+            uint32_t x = cin;
+            for (int i = 0; i < CARRY_WAVES; ++i) {
+                wave_in[i] = x;
+                x = waves[i].g | (waves[i].p & x);
+                b = compose(b, waves[i]);
+            }
+            C.agg[chunk] = (uint8_t)(b.g | (b.p << 1) | (cin << 2) | (resolved << 3));
         }
     }
     __syncthreads();
@@ -230,14 +319,60 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_apply(const CarryLaunch
     for (int it = 0; it < CARRY_ITERS; ++it) {
         const uint64_t sum = G[it].X + G[it].Y + cin;
         const uint32_t x = (uint32_t)(((sum ^ G[it].X ^ G[it].Y) >> lane) & 1);
-        const size_t j = base + 64 * it;
-        if (j < N.out_len) {
-            N.out[j] = t[it] + x;
+        const long j = base + 64 * it;
+        if ((size_t)j < N.out_len) {
+            N.out[j] = t[it + 1] + x;
             if (N.out_hi) N.out_hi[j] = 0;
         }
-        if (j == N.m && N.flag) *N.flag = t[it] + x;
+        if ((size_t)j == N.m && N.flag) *N.flag = t[it + 1] + x;
         cin = group_out(G[it], cin);
     }
+}
+
+// One wave per chunk: an unresolved chunk whose true carry-in (from the scan)
+// is 1 was written with 0; add 1 at its lowest limb, rippling through the
+// limbs that propagate (written as 2^64 - 1, now 0) into the first that
+// does not (written as t, now t + 1).  Limbs past out_len are not stored
+// but still carry the ripple; limb m is the flag word.
+template <bool PLAIN>
+__global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C) {
+    const size_t chunk = (size_t)blockIdx.x * CARRY_WAVES + (threadIdx.x >> 6);
+    if (chunk >= C.nchunks) return;
+    const uint32_t v = C.agg[chunk];
+    if ((v & 8) || !C.cin[chunk]) return;
+    const CarryNode &N = PLAIN ? C.one : node_of(C, chunk);
+    const int lane = threadIdx.x & 63;
+    const long cbase = (long)(chunk - N.chunk0) * CARRY_CHUNK;
+    for (int grp = 0; grp < CARRY_CHUNK / 64; ++grp) {
+        const long j = cbase + 64 * grp + lane;
+        uint64_t t[1];
+        uint32_t g[1], p[1];
+        chunk_limbs<PLAIN, 1>(N, j, t, g, p);
+        const uint64_t stop = __ballot(!p[0]);
+        const int k = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
+        if (lane <= k) {   // k == 64: the whole group propagates
+            const uint64_t val = t[0] + 1;
+            if ((size_t)j < N.out_len) N.out[j] = val;
+            if ((size_t)j == N.m && N.flag) *N.flag = val;
+        }
+        if (stop) return;
+    }
+}
+
+hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream) {
+    if (!C.nchunks) return hipSuccess;
+    if (!C.agg || !C.cin) return hipErrorInvalidValue;
+    const unsigned nch = (unsigned)C.nchunks, nfix = (unsigned)((C.nchunks + CARRY_WAVES - 1) / CARRY_WAVES);
+    if (!C.nodes && C.one.kind == NODE_PLAIN && C.one.parts == 1) {
+        hipLaunchKernelGGL(k_carry_spec<true>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
+        hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
+        hipLaunchKernelGGL(k_carry_fix<true>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+    } else {
+        hipLaunchKernelGGL(k_carry_spec<false>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
+        hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
+        hipLaunchKernelGGL(k_carry_fix<false>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace pg

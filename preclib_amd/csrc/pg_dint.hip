@@ -100,11 +100,8 @@ int lin_run(const CarryNode *nodes, size_t n, const char *stage) {
     cl.cin = (uint8_t *)flags.d + nch;
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
-    hipLaunchKernelGGL(k_carry_reduce, dim3(nch), dim3(CARRY_THREADS), 0, C.stream, cl);
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, C.stream, cl);
-    hipLaunchKernelGGL(k_carry_apply, dim3(nch), dim3(CARRY_THREADS), 0, C.stream, cl);
+    HIP_TRY(launch_carry_kernels(cl, C.stream));
     C.stage_end(stage, e0, bytes);
-    HIP_TRY(hipGetLastError());
     return PG_OK;
 }
 

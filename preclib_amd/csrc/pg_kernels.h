@@ -264,12 +264,13 @@ struct CarryLaunch {
     const CarryNode *nodes;
     const uint32_t *chunk_node; // node of every chunk
     size_t nchunks;
-    uint8_t *agg;               // per-chunk (G | P << 1)
-    uint8_t *cin;               // per-chunk carry-in
+    uint8_t *agg;               // per-chunk G | P << 1 | used carry-in << 2 | resolved << 3
+    uint8_t *cin;               // per-chunk true carry-in (from the scan)
 };
 
-__global__ void k_carry_reduce(const CarryLaunch C);
 __global__ void k_carry_scan(const CarryLaunch C);
-__global__ void k_carry_apply(const CarryLaunch C);
+// Enqueues the carry of C on `stream`: the speculative pass, the scan of
+// chunk aggregates and the fix-up pass (pg_carry.hip).
+hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream);
 
 }  // namespace pg

@@ -69,12 +69,13 @@ __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
             for (int j = 0; j < h; ++j) {
                 uint32_t u = x[b + j], v = x[b + j + h];
                 x[b + j] = mod_add(u, v, p);
-                uint32_t d = mod_sub(u, v, p);
                 if (j == 0) {
-                    x[b + j + h] = d;
+                    x[b + j + h] = mod_sub(u, v, p);
                 } else {
+                    // u - v + p lies in [1, 2p) < 2^32: Shoup's product takes
+                    // it unreduced
                     const int k = j * (16 / h);   // omega_{2h}^j = omega_32^(j*16/h)
-                    x[b + j + h] = mul_shoup(d, c.w32[k], c.w32q[k], p);
+                    x[b + j + h] = mul_shoup(u - v + p, c.w32[k], c.w32q[k], p);
                 }
             }
         }

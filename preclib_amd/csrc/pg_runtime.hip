@@ -442,11 +442,8 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
     if (!cl.agg || !cl.cin) return set_error(PG_ENOMEM, "carry scratch");
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
-    hipLaunchKernelGGL(k_carry_reduce, dim3(cl.nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, C.stream, cl);
-    hipLaunchKernelGGL(k_carry_apply, dim3(cl.nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
+    HIP_TRY(launch_carry_kernels(cl, C.stream));
     C.stage_end("carry", e0, (double)m * 32.0 + (double)out_len * 8.0);
-    HIP_TRY(hipGetLastError());
     return PG_OK;
 }
 
@@ -513,11 +510,8 @@ int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dno
     for (auto &n : nodes) bytes += (double)n.m * 16.0 * (n.kind == NODE_BLOCKED ? 2 : 1) + (double)n.out_len * 8.0;
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
-    hipLaunchKernelGGL(k_carry_reduce, dim3(nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, C.stream, cl);
-    hipLaunchKernelGGL(k_carry_apply, dim3(nchunks), dim3(CARRY_THREADS), 0, C.stream, cl);
+    HIP_TRY(launch_carry_kernels(cl, C.stream));
     C.stage_end("carry", e0, bytes);
-    HIP_TRY(hipGetLastError());
     return PG_OK;
 }
 
