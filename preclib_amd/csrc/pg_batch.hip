@@ -377,6 +377,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
             L.logn = b.logn;
             L.mode = b.mode;
             L.variant = ctx().ntt_variant(b.logn, b.mode);
+            L.order = ctx().ntt_order;
             int st = fill_ntt_tables(L);
             if (st) return st;
         } else {

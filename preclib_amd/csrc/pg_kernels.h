@@ -81,8 +81,18 @@ constexpr bool plan_distinct(int logn, int v) {
     return v == 0 || plan_l2(logn, v) != plan_l2(logn) || plan_lt(logn, v) != plan_lt(logn) ||
            plan_rb(logn, v) != plan_rb(logn);
 }
-// Row workgroup: 512 threads, 1024 for 2^15-point rows (32 kept values each).
-constexpr int row_threads(int logn, int v = 0) { return plan_l2(logn, v) >= 15 ? 1024 : 512; }
+// Row workgroup: 512 threads; 1024 for 2^15-point rows (32 kept values each)
+// and for radix-8 plans of 2^13 points and up (8 kept values each and every
+// thread busy in every pass, in at most 64 VGPRs: two workgroups per CU).
+constexpr int row_threads(int logn, int v = 0) {
+    return (plan_l2(logn, v) >= 15 || (plan_rb(logn, v) == 3 && plan_l2(logn, v) >= 13)) ? 1024 : 512;
+}
+// Minimum waves per SIMD asked of the compiler for the row kernel: 8 (64
+// VGPRs, two 1024-thread workgroups per CU) for radix-8 rows of 8 values per
+// thread; otherwise no request beyond what the workgroup size needs.
+constexpr int row_waves(int logn, int v = 0) {
+    return (row_threads(logn, v) == 1024 && plan_rb(logn, v) == 3 && (1 << plan_l2(logn, v)) <= 8 * 1024) ? 8 : 1;
+}
 
 // Digit modes of the NTT product.
 //  MODE_REF  : preclib's own decomposition (mul_ntt.cpp:331-342, 509-513):
@@ -93,6 +103,10 @@ constexpr int row_threads(int logn, int v = 0) { return plan_l2(logn, v) >= 15 ?
 //              the NTT path accepts (<= 2^24 limbs), so the three-prime CRT is
 //              exact; the transforms are half as long with 3/2 as many primes.
 enum { MODE_REF = 0, MODE_WIDE = 1 };
+// Forward column kernel grids: (tile, operand x prime), (tile x prime,
+// operand) with a tile's primes adjacent on one XCD, (tile x operand, prime)
+// with a tile's operands adjacent, and the same with primes last to first.
+enum { COLF_OP_PRIME = 0, COLF_TILE_PRIME = 1, COLF_TILE_OP = 2, COLF_TILE_OP_REV = 3, ROW_PRIME_REV = 4 };
 constexpr int mode_primes(int mode) { return mode == MODE_WIDE ? 3 : 2; }
 constexpr int mode_digit_bits(int mode) { return mode == MODE_WIDE ? 32 : 16; }
 
@@ -124,6 +138,8 @@ struct NttLaunch {
     double io_bytes;            // operand + product limb bytes of all leaves (timing model)
     int logn, lobits, nprimes, mode;
     int variant;                // plan variant (wide mode), chosen by the autotuner table
+    int order;                  // workgroup order: bits 0-1 forward column kernel (COLF_*), bit 2
+                                // row kernel primes last to first
     PrimeK pk[3];
     CrtK crt;
     const uint2 *col_tw[3][MAX_PASS], *col_itw[3][MAX_PASS];

@@ -50,15 +50,24 @@ __device__ __forceinline__ void lds_sync() {
 #endif
 }
 
+// threadIdx.x through an empty asm: a value the compiler cannot reuse across
+// calls, so the index arithmetic of each transform is computed where it is
+// used instead of being held in registers from one transform to the next.
+__device__ __forceinline__ int fresh_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // ---- in-LDS radix passes --------------------------------------------------
 
 template <int LR, int LOGN, int LOGT, int LOGB, int NT>
-__device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict__ tw, const PrimeK &c) {
+__device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict__ tw, const PrimeK &c, int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
     constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
     constexpr int S = M * T;
 #pragma unroll 1
-    for (int col = threadIdx.x; col < NCOLS; col += NT) {
+    for (int col = tid; col < NCOLS; col += NT) {
         const int cc = col & (T - 1), r = col >> LOGT;
         const int blk = r >> LOGM, j2 = r & (M - 1);
         const int base = (((blk << LOGB) + j2) << LOGT) + cc;
@@ -80,12 +89,12 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
 }
 
 template <int LR, int LOGN, int LOGT, int LOGB, int NT>
-__device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict__ itw, const PrimeK &c) {
+__device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict__ itw, const PrimeK &c, int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
     constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
     constexpr int S = M * T;
 #pragma unroll 1
-    for (int col = threadIdx.x; col < NCOLS; col += NT) {
+    for (int col = tid; col < NCOLS; col += NT) {
         const int cc = col & (T - 1), r = col >> LOGT;
         const int blk = r >> LOGM, j2 = r & (M - 1);
         const int base = (((blk << LOGB) + j2) << LOGT) + cc;
@@ -109,33 +118,37 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
 // Pass lists: forward runs passes 0..np-1 with shrinking block size; the
 // inverse runs them backwards (post-order recursion).
 template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB>
-__device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c) {
+__device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0) {
-        pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c);
+        pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c, tid);
         lds_sync();
-        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, tw, c);
+        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, tw, c, tid);
     }
 }
 
 template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB>
-__device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c) {
+__device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0) {
-        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, itw, c);
-        pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c);
+        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, itw, c, tid);
+        pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c, tid);
         lds_sync();
     }
 }
 
+// tid: the thread index (callers may pass an opaque copy of threadIdx.x so
+// that index arithmetic is recomputed per call rather than kept live).
 template <int LOGN, int LOGT, int NT, int RB>
-__device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c) {
-    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, tw, c);
+__device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c,
+                                            int tid = threadIdx.x) {
+    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, tw, c, tid);
 }
 
 template <int LOGN, int LOGT, int NT, int RB>
-__device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c) {
-    inv_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, itw, c);
+__device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c,
+                                            int tid = threadIdx.x) {
+    inv_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, itw, c, tid);
 }
 
 // Frequency index held at position rho after the forward passes of a 2^LOGN
@@ -160,11 +173,41 @@ __device__ __forceinline__ uint32_t twiddle_mont(const uint32_t *__restrict__ lo
     return mont_mul(lo[e & ((1u << lobits) - 1)], hi[e >> lobits], c.p, c.ninv);
 }
 
+// Four-step twiddles of a column tile.  Thread t holds elements
+// e = t + k * NT (k < E) of the tile: row rho = e >> LT, column
+// C = c0 + (t mod T), twiddle omega_n^(C * freq(rho)).  When the rows of one
+// thread differ only in the top radix digit (the first pass's), freq(rho)
+// steps by 2^D from one element to the next, so each twiddle is the previous
+// one times omega_n^(C * 2^D): one Montgomery product per element instead of
+// forming each from the tables.  Otherwise every twiddle is formed directly.
+template <int L1, int LT, int RBV, int NT>
+struct ColTw {
+    static constexpr int LNT = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : -1;
+    static constexpr int E = ((1 << (L1 + LT)) + NT - 1) / NT;
+    static constexpr int LR0 = split_lr(L1, 0, RBV);
+    static constexpr bool STEP = LNT > 0 && E > 1 && (LNT - LT) >= (L1 - LR0);
+    static constexpr int D = STEP ? (LNT - LT) - (L1 - LR0) : 0;
+};
+
 // Workgroups are dealt to the 8 XCDs round-robin by dispatch order; mapping
 // workgroup i to tile (i % 8) * (ntiles / 8) + i / 8 keeps neighbouring tiles
 // (which share 128-byte lines of every row) on one XCD's L2.
 __device__ __forceinline__ int xcd_tile(int i, int ntiles) {
     return (ntiles & 7) ? i : (i & 7) * (ntiles >> 3) + (i >> 3);
+}
+
+// Column-kernel workgroup i of ntiles * np: tile and prime.  Consecutive
+// workgroups of one XCD (i, i + 8, i + 16, ...) take the primes of one tile,
+// then the next tile; each XCD covers a contiguous run of tiles.
+__device__ __forceinline__ void xcd_tile_prime(int i, int ntiles, int np, int &tile, int &pr) {
+    if (ntiles & 7) {
+        tile = i / np;
+        pr = i % np;
+        return;
+    }
+    const int slot = i >> 3;
+    pr = slot % np;
+    tile = (i & 7) * (ntiles >> 3) + slot / np;
 }
 
 // x mod p for any x < 2^32 (Shoup's product by 1 with quotient floor(2^32/p)).
@@ -189,36 +232,73 @@ __device__ __forceinline__ void put_digits(uint32_t *d, uint64_t v, uint32_t red
 
 // ---- kernel 1: forward column pass ------------------------------------------
 
+// One workgroup per (tile, prime, operand), in the order L.order selects
+// (COLF_*): with COLF_TILE_PRIME the workgroups of one tile's primes are dealt
+// to the same XCD one after another (xcd_tile_prime), so the operand limbs
+// that the first reads from HBM the others find in L2.
 template <int LOGN, int MODE, int V>
-__global__ void __launch_bounds__(NTT_THREADS) k_col_fwd(const NttLaunch L) {
-    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+__global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(8))) k_col_fwd(const NttLaunch L) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
     constexpr int DPL = 64 / mode_digit_bits(MODE);   // digits per limb
     constexpr int N1 = 1 << L1, T = 1 << LT, LPR = T / DPL;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     constexpr int NP = mode_primes(MODE);
+    using TW = ColTw<L1, LT, RBV, NTT_THREADS>;
     extern __shared__ uint32_t lds[];
     const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x), op = blockIdx.y / NP, pr = blockIdx.y % NP;
-    const PrimeK &c = L.pk[pr];
+    int tile, pr, op;
+    switch (L.order & 3) {
+    case COLF_OP_PRIME:
+        tile = xcd_tile(blockIdx.x, gridDim.x);
+        op = blockIdx.y / NP;
+        pr = blockIdx.y % NP;
+        break;
+    case COLF_TILE_PRIME:
+        xcd_tile_prime(blockIdx.x, gridDim.x / NP, NP, tile, pr);
+        op = blockIdx.y;
+        break;
+    default: {
+        const int t2 = xcd_tile(blockIdx.x, gridDim.x);
+        tile = t2 >> 1;
+        op = t2 & 1;
+        pr = (L.order & 3) == COLF_TILE_OP ? blockIdx.y : NP - 1 - blockIdx.y;
+    }
+    }
     const uint32_t c0 = (uint32_t)tile << LT;
     const uint64_t *src = op ? D.b : D.a;
     const size_t nl = op ? D.bn : D.an;
-    // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T)
-    for (int e = threadIdx.x; e < N1 * LPR; e += NTT_THREADS) {
-        const int j1 = e / LPR, m = e % LPR;
-        const size_t li = (((size_t)j1 << L2) + c0 + DPL * m) / DPL;
-        // DPL aligned words never straddle a pad
-        put_digits<MODE>(lds + lp(j1 * T + DPL * m), li < nl ? src[li] : 0, L.crt.red[pr], c.p);
-    }
-    lds_sync();
-    lds_forward<L1, LT, NTT_THREADS, plan_rb(LOGN, V)>(lds, L.col_tw[pr], c);
-    uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
-    for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
-        const int rho = e >> LT, cc = e & (T - 1);
-        const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;
-        uint32_t x = lds[lp(e)];
-        if (ex) x = mont_mul(x, twiddle_mont(L.glo[pr], L.ghi[pr], ex, L.lobits, c), c.p, c.ninv);
-        out[((size_t)rho << L2) + c0 + cc] = x;
+    const uint32_t C = c0 + (threadIdx.x & (T - 1));
+    {
+        const PrimeK &c = L.pk[pr];
+        // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T)
+        for (int e = threadIdx.x; e < N1 * LPR; e += NTT_THREADS) {
+            const int j1 = e / LPR, m = e % LPR;
+            const size_t li = (((size_t)j1 << L2) + c0 + DPL * m) / DPL;
+            // DPL aligned words never straddle a pad
+            put_digits<MODE>(lds + lp(j1 * T + DPL * m), li < nl ? src[li] : 0, L.crt.red[pr], c.p);
+        }
+        lds_sync();
+        lds_forward<L1, LT, NTT_THREADS, RBV>(lds, L.col_tw[pr], c);
+        uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
+        if constexpr (TW::STEP) {
+            const uint32_t f0 = freq_of_pos<L1, RBV>(threadIdx.x >> LT);
+            uint32_t w = twiddle_mont(L.glo[pr], L.ghi[pr], (C * f0) & NMASK, L.lobits, c);
+            const uint32_t st = twiddle_mont(L.glo[pr], L.ghi[pr], (C << TW::D) & NMASK, L.lobits, c);
+#pragma unroll 4
+            for (int k = 0; k < TW::E; ++k) {
+                const int e = threadIdx.x + k * NTT_THREADS;
+                out[((size_t)(e >> LT) << L2) + C] = mont_mul(lds[lp(e)], w, c.p, c.ninv);
+                w = mont_mul(w, st, c.p, c.ninv);
+            }
+        } else {
+            for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
+                const int rho = e >> LT, cc = e & (T - 1);
+                const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, RBV>(rho)) & NMASK;
+                uint32_t x = lds[lp(e)];
+                if (ex) x = mont_mul(x, twiddle_mont(L.glo[pr], L.ghi[pr], ex, L.lobits, c), c.p, c.ninv);
+                out[((size_t)rho << L2) + c0 + cc] = x;
+            }
+        }
     }
 }
 
@@ -247,17 +327,19 @@ __device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, cons
 }
 
 template <int LOGN, int MODE, int V>
-__global__ void __launch_bounds__(row_threads(LOGN, V)) k_row(const NttLaunch L) {
+__global__ void __launch_bounds__(row_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(row_waves(LOGN, V))))
+k_row(const NttLaunch L) {
     constexpr int L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
     constexpr int E = (N2 + ROW_THREADS - 1) / ROW_THREADS;
     extern __shared__ uint32_t lds[];
     const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
-    const int row = blockIdx.x, pr = blockIdx.y;
+    const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
     const PrimeK &c = L.pk[pr];
     uint32_t keep[E];
 
     constexpr int E4 = (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS;
-    constexpr bool PF = plan_l1(LOGN, V) > 0 && E4 <= 4;   // larger rows would spill
+    // larger rows would spill; radix-8 rows at 64 VGPRs rely on occupancy instead
+    constexpr bool PF = plan_l1(LOGN, V) > 0 && E4 <= 4 && row_waves(LOGN, V) < 8;
     load_row<LOGN, MODE, V>(lds, L, D, 0, pr, row);
     // operand B's spectrum row is fetched now and lands during A's transform
     uint4 pb[PF ? E4 : 1];
@@ -318,201 +400,190 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t v, i
     lo = s;
 }
 
-// 128-bit accumulate: (lo, hi) += a * b, shifted left by 32 bits when up32.
-__device__ __forceinline__ void acc_mul128(uint64_t &lo, uint64_t &hi, uint64_t a, uint32_t b, bool up32) {
-    uint64_t plo = a * b, phi = __umul64hi(a, (uint64_t)b);
-    if (up32) {
-        phi = (phi << 32) | (plo >> 32);
-        plo <<= 32;
-    }
-    const uint64_t s = lo + plo;
-    hi += phi + (s < lo);
-    lo = s;
-}
-
-// Prime pr's residues of the tile into registers: element e = threadIdx.x +
-// k * COLINV_THREADS of the tile (row e >> LT, column e & (T - 1)).
-template <int LOGN, int V, int E>
-__device__ __forceinline__ void load_column(uint32_t (&pf)[E], const NttLeaf &D, int pr, uint32_t c0) {
-    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
-    constexpr int T = 1 << LT, NE = (1 << L1) * T;
-    const uint32_t *in = D.prod + ((size_t)pr << LOGN);
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int e = threadIdx.x + k * COLINV_THREADS;
-        if (e < NE) pf[k] = in[((size_t)(e >> LT) << L2) + c0 + (e & (T - 1))];
-    }
-}
-
-// Inverse twiddle and 1/n of prime pr's prefetched residues into LDS.
-template <int LOGN, int V, int E>
-__device__ __forceinline__ void stage_column(uint32_t *lds, const uint32_t (&pf)[E], const NttLaunch &L, int pr,
-                                             uint32_t c0) {
-    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V);
+// Prime pr's product residues of the tile, times the inverse four-step
+// twiddle omega_n^(-C * freq(rho)) and 1/n (both folded into the ilo / ihi
+// tables), into LDS.  Element e = threadIdx.x + k * COLINV_THREADS: row
+// e >> LT, column c0 + (e mod T).
+template <int LOGN, int V>
+__device__ __forceinline__ void stage_prime(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int pr,
+                                            uint32_t c0, int tid) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
     constexpr int T = 1 << LT, NE = (1 << L1) * T;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
+    using TW = ColTw<L1, LT, RBV, COLINV_THREADS>;
     const PrimeK &c = L.pk[pr];
+    const uint32_t *in = D.prod + ((size_t)pr << LOGN);
+    const uint32_t C = c0 + (tid & (T - 1));
+    if constexpr (TW::STEP) {
+        const uint32_t f0 = freq_of_pos<L1, RBV>(tid >> LT);
+        uint32_t w = twiddle_mont(L.ilo[pr], L.ihi[pr], (C * f0) & NMASK, L.lobits, c);
+        // omega_n^(-C * 2^D) in Montgomery form, from the forward tables
+        const uint32_t st = twiddle_mont(L.glo[pr], L.ghi[pr], (0u - (C << TW::D)) & NMASK, L.lobits, c);
+        // residues in groups of 4 loads in flight
+        constexpr int G = TW::E < 4 ? TW::E : 4;
+#pragma unroll 1
+        for (int k0 = 0; k0 < TW::E; k0 += G) {
+            uint32_t x[G];
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int e = threadIdx.x + k * COLINV_THREADS;
-        if (e < NE) {
-            const int rho = e >> LT, cc = e & (T - 1);
-            const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;   // ilo/ihi hold omega^-e
-            lds[lp(e)] = mont_mul(pf[k], twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
+            for (int k = 0; k < G; ++k) {
+                const int e = tid + (k0 + k) * COLINV_THREADS;
+                x[k] = in[((size_t)(e >> LT) << L2) + C];
+            }
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                lds[lp(tid + (k0 + k) * COLINV_THREADS)] = mont_mul(x[k], w, c.p, c.ninv);
+                w = mont_mul(w, st, c.p, c.ninv);
+            }
         }
-    }
-}
-
-// Inverse column transform of prime pr for the tile (result left in LDS);
-// the residues of prime pr + 1 are fetched into pf meanwhile.
-// Tiles of at most 8 residues per thread keep the next prime's residues in
-// flight during the transform; larger ones load each prime just before use
-// (registers would spill).
-template <int E>
-constexpr bool col_prefetch() { return E <= 8; }
-
-template <int LOGN, int V, int E>
-__device__ __forceinline__ void inv_column(uint32_t *lds, uint32_t (&pf)[E], const NttLaunch &L, const NttLeaf &D,
-                                           int pr, int np, uint32_t c0) {
-    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
-    if constexpr (col_prefetch<E>()) {
-        stage_column<LOGN, V, E>(lds, pf, L, pr, c0);
-    } else {   // load, twiddle and store one residue at a time
-        constexpr int T = 1 << LT, NE = (1 << L1) * T;
-        constexpr uint32_t NMASK = (1u << LOGN) - 1;
-        const PrimeK &c = L.pk[pr];
-        const uint32_t *in = D.prod + ((size_t)pr << LOGN);
-        for (int e = threadIdx.x; e < NE; e += COLINV_THREADS) {
+    } else {
+        for (int e = tid; e < NE; e += COLINV_THREADS) {
             const int rho = e >> LT, cc = e & (T - 1);
-            const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, plan_rb(LOGN, V)>(rho)) & NMASK;
+            const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, RBV>(rho)) & NMASK;
             const uint32_t x = in[((size_t)rho << L2) + c0 + cc];
             lds[lp(e)] = mont_mul(x, twiddle_mont(L.ilo[pr], L.ihi[pr], ex, L.lobits, c), c.p, c.ninv);
         }
     }
-    lds_sync();
-    if constexpr (col_prefetch<E>())
-        if (pr + 1 < np) load_column<LOGN, V, E>(pf, D, pr + 1, c0);
-    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS, plan_rb(LOGN, V)>(lds, L.col_itw[pr], L.pk[pr]);
 }
 
-template <int LOGN, int MODE, int V>
-__global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
+// Prime pr's residues through the inverse column transform; the result is in
+// LDS (after a workgroup barrier).
+template <int LOGN, int V>
+__device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int pr, uint32_t c0,
+                                           int tid = threadIdx.x) {
+    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V);
+    stage_prime<LOGN, V>(lds, L, D, pr, c0, tid);
+    lds_sync();
+    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS, plan_rb(LOGN, V)>(lds, L.col_itw[pr], L.pk[pr], tid);
+}
+
+// Value of lane (this lane ^ 1), by a DPP quad permutation [1, 0, 3, 2].
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+
+// Wide mode: three primes, Garner
+//   t2 = (r2 - r1) p1^-1 mod p2,  t3 = (r3 - (r1 + p1 t2)) (p1 p2)^-1 mod p3,
+//   x  = r1 + p1 t2 + p1 p2 t3 < 2^89 (the exact coefficient);
+// a limb is columns (2m, 2m+1) of a row, x_even + 2^32 x_odd, held by lanes
+// (2i, 2i+1), so the odd lane's coefficient reaches its partner by DPP and the
+// even lane writes the 128-bit limb sum.  At most 64 VGPRs, so two
+// workgroups share a CU.
+template <int LOGN, int V>
+__device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, uint32_t c0) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
-    constexpr int DPL = 64 / mode_digit_bits(MODE);
+    constexpr int T = 1 << LT, NE = (1 << L1) * T;
+    constexpr int E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
+    const CrtK &K = L.crt;
+    uint32_t r0[E], t2[E];
+#pragma unroll 1
+    for (int pr = 0; pr < 3; ++pr) {
+        if (pr) lds_sync();   // the previous prime's residues have been read
+        // The thread index is re-read each prime, so the index arithmetic of
+        // the passes is recomputed instead of being held, beside the Garner
+        // state, across the whole loop.
+        const int tid = fresh_tid();
+        const uint32_t C = c0 + (tid & (T - 1));
+        const bool even = (tid & 1) == 0;
+        uint32_t *lds_p = lds;
+        inv_column<LOGN, V>(lds_p, L, D, pr, c0, tid);
+        if (pr == 0) {
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int e = tid + k * COLINV_THREADS;
+                r0[k] = e < NE ? lds_p[lp(e)] : 0;
+            }
+        } else if (pr == 1) {
+            const uint32_t p2 = L.pk[1].p;
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int e = tid + k * COLINV_THREADS;
+                const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
+                t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, p2);
+            }
+        } else {
+            const uint32_t p3 = L.pk[2].p;
+            const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
+            const uint32_t p12lo = (uint32_t)p12, p12hi = (uint32_t)(p12 >> 32);
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int e = tid + k * COLINV_THREADS;
+                const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
+                // (r1 + p1 t2) mod p3, with r1 < p1 < p3 and t2 < p2 < p3
+                const uint32_t x12m = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, p3), p3);
+                const uint32_t t3 = mul_shoup(mod_sub(r, x12m, p3), K.inv123, K.inv123q, p3);
+                // x = a + 2^32 b
+                const uint64_t x12 = (uint64_t)r0[k] + p1 * t2[k];
+                const uint64_t s0 = x12 + (uint64_t)p12lo * t3;
+                const uint32_t a = (uint32_t)s0;
+                const uint64_t b = (s0 >> 32) + ((uint64_t)(s0 < x12) << 32) + (uint64_t)p12hi * t3;
+                const uint32_t ao = pair_swap(a);
+                const uint32_t bo_lo = pair_swap((uint32_t)b), bo_hi = pair_swap((uint32_t)(b >> 32));
+                if (even && e < NE) {
+                    const uint64_t m = b + ao;
+                    const size_t li = ((size_t)(e >> LT) << (L2 - 1)) + (C >> 1);
+                    if (li < D.sum_len) {
+                        D.slo[li] = (uint64_t)a | (m << 32);
+                        D.shi[li] = (m >> 32) + ((uint64_t)bo_hi << 32 | bo_lo);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Reference mode: two primes, coefficient x = r1 + p1 t (< 2^60), four
+// coefficients per limb, summed through LDS.
+template <int LOGN, int V>
+__device__ __forceinline__ void col_inv_ref(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, uint32_t c0) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int DPL = 4;
     constexpr int N1 = 1 << L1, T = 1 << LT, NE = N1 * T, LPR = T / DPL, NL = N1 * LPR;
     constexpr int E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
     constexpr int EL = (NL + COLINV_THREADS - 1) / COLINV_THREADS;
-    extern __shared__ uint32_t lds[];
-    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t c0 = (uint32_t)tile << LT;
     const CrtK &K = L.crt;
     uint64_t slo[EL], shi[EL];
 #pragma unroll
     for (int k = 0; k < EL; ++k) { slo[k] = 0; shi[k] = 0; }
-
-    if constexpr (MODE == MODE_REF) {
-        // Two primes; coefficient x = r1 + p1 t (< 2^60), 4 coefficients per limb.
-        uint32_t res0[E];
-        uint64_t coef[E];
-        uint32_t pf[E];
-        if constexpr (col_prefetch<E>()) load_column<LOGN, V, E>(pf, D, 0, c0);
-        for (int pr = 0; pr < 2; ++pr) {
-            inv_column<LOGN, V, E>(lds, pf, L, D, pr, 2, c0);
+    uint32_t res0[E];
+    uint64_t coef[E];
+    for (int pr = 0; pr < 2; ++pr) {
+        inv_column<LOGN, V>(lds, L, D, pr, c0);
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int i = threadIdx.x + k * COLINV_THREADS;
-                if (i < NE) {
-                    const uint32_t r = lds[lp(i)];
-                    if (pr == 0) {
-                        res0[k] = r;
-                    } else {
-                        // Garner (mul_ntt.cpp:405-412): x = r1 + p1 * ((r2 - r1) * p1^-1 mod p2)
-                        const uint32_t d = mod_sub(r, res0[k], L.pk[1].p);   // r1 < p1 < p2
-                        const uint32_t t = mul_shoup(d, K.inv12, K.inv12q, L.pk[1].p);
-                        coef[k] = (uint64_t)res0[k] + (uint64_t)L.pk[0].p * t;
-                    }
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * COLINV_THREADS;
+            if (i < NE) {
+                const uint32_t r = lds[lp(i)];
+                if (pr == 0) {
+                    res0[k] = r;
+                } else {
+                    // Garner (mul_ntt.cpp:405-412): x = r1 + p1 * ((r2 - r1) * p1^-1 mod p2)
+                    const uint32_t d = mod_sub(r, res0[k], L.pk[1].p);   // r1 < p1 < p2
+                    const uint32_t t = mul_shoup(d, K.inv12, K.inv12q, L.pk[1].p);
+                    coef[k] = (uint64_t)res0[k] + (uint64_t)L.pk[0].p * t;
                 }
             }
-            lds_sync();
         }
+        lds_sync();
+    }
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
+    for (int half = 0; half < 2; ++half) {
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int i = threadIdx.x + k * COLINV_THREADS;
-                if (i < NE) lds[lp(i)] = half ? (uint32_t)(coef[k] >> 32) : (uint32_t)coef[k];
-            }
-            lds_sync();
-#pragma unroll
-            for (int k = 0; k < EL; ++k) {
-                const int e = threadIdx.x + k * COLINV_THREADS;
-                if (e < NL) {
-                    const int rho = e / LPR, m = e % LPR;
-                    const uint32_t *w = lds + lp(rho * T + 4 * m);
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) acc128(slo[k], shi[k], w[t], 16 * t + 32 * half);
-                }
-            }
-            lds_sync();
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * COLINV_THREADS;
+            if (i < NE) lds[lp(i)] = half ? (uint32_t)(coef[k] >> 32) : (uint32_t)coef[k];
         }
-    } else {
-        // Three primes, Garner: t2 = (r2 - r1) p1^-1 mod p2,
-        // t3 = (r3 - (r1 + p1 t2)) (p1 p2)^-1 mod p3,
-        // x = r1 + p1 t2 + p1 p2 t3 < 2^91; 2 coefficients per limb.
-        uint32_t r0[E], t2[E];
-        uint32_t pf[E];
-        if constexpr (col_prefetch<E>()) load_column<LOGN, V, E>(pf, D, 0, c0);
-        for (int pr = 0; pr < 3; ++pr) {
-            inv_column<LOGN, V, E>(lds, pf, L, D, pr, 3, c0);
+        lds_sync();
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int i = threadIdx.x + k * COLINV_THREADS;
-                if (i < NE) {
-                    const uint32_t r = lds[lp(i)];
-                    if (pr == 0) {
-                        r0[k] = r;
-                    } else if (pr == 1) {
-                        const uint32_t p2 = L.pk[1].p;
-                        t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, p2);
-                    } else {
-                        const uint32_t p3 = L.pk[2].p;
-                        // (r1 + p1 t2) mod p3, with r1 < p1 < p3 and t2 < p2 < p3
-                        const uint32_t x12 = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, p3), p3);
-                        lds[lp(i)] = mul_shoup(mod_sub(r, x12, p3), K.inv123, K.inv123q, p3);   // t3
-                    }
-                }
+        for (int k = 0; k < EL; ++k) {
+            const int e = threadIdx.x + k * COLINV_THREADS;
+            if (e < NL) {
+                const int rho = e / LPR, m = e % LPR;
+                const uint32_t *w = lds + lp(rho * T + 4 * m);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc128(slo[k], shi[k], w[t], 16 * t + 32 * half);
             }
-            lds_sync();
         }
-        // LDS now holds t3.  The limb's value is sum_j x_j 2^(32 j) over its
-        // columns (2m, 2m+1) with x = r1 + p1 t2 + p1 p2 t3, linear in the
-        // three words, so each word is added into the 128-bit sum as it
-        // arrives: t3 times p1 p2, then r1, then t2 times p1.
-        const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
-#pragma unroll
-        for (int word = 0; word < 3; ++word) {
-            if (word > 0) {
-#pragma unroll
-                for (int k = 0; k < E; ++k) {
-                    const int i = threadIdx.x + k * COLINV_THREADS;
-                    if (i < NE) lds[lp(i)] = word == 1 ? r0[k] : t2[k];
-                }
-                lds_sync();
-            }
-            const uint64_t mult = word == 0 ? p12 : word == 1 ? 1 : p1;
-#pragma unroll
-            for (int k = 0; k < EL; ++k) {
-                const int e = threadIdx.x + k * COLINV_THREADS;
-                if (e < NL) {
-                    const int rho = e / LPR, m = e % LPR;
-                    const uint32_t *w = lds + lp(rho * T + 2 * m);
-                    acc_mul128(slo[k], shi[k], mult, w[0], false);
-                    acc_mul128(slo[k], shi[k], mult, w[1], true);
-                }
-            }
-            lds_sync();
-        }
+        lds_sync();
     }
 #pragma unroll
     for (int k = 0; k < EL; ++k) {
@@ -528,6 +599,24 @@ __global__ void __launch_bounds__(COLINV_THREADS) k_col_inv(const NttLaunch L) {
     }
 }
 
+// Two workgroups per CU (at most 64 VGPRs) where the Garner state (2 words per
+// element) and a transform pass fit: 8 elements per thread or radix-8 passes.
+template <int LOGN, int V>
+constexpr int col_inv_waves() {
+    return ((1 << (plan_l1(LOGN, V) + plan_lt(LOGN, V))) <= 8 * COLINV_THREADS || plan_rb(LOGN, V) == 3) ? 8 : 4;
+}
+
+template <int LOGN, int MODE, int V>
+__global__ void __launch_bounds__(COLINV_THREADS)
+__attribute__((amdgpu_waves_per_eu(MODE == MODE_WIDE ? col_inv_waves<LOGN, V>() : 1)))
+k_col_inv(const NttLaunch L) {
+    extern __shared__ uint32_t lds[];
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const uint32_t c0 = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) << plan_lt(LOGN, V);
+    if constexpr (MODE == MODE_WIDE) col_inv_wide<LOGN, V>(lds, L, D, c0);
+    else col_inv_ref<LOGN, V>(lds, L, D, c0);
+}
+
 // ---- host-side launch table ----------------------------------------------------
 
 template <int LOGN, int MODE, int V>
@@ -540,7 +629,10 @@ static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const Stag
     const double n = (double)(1u << LOGN) * nl, io = L.io_bytes;
     if constexpr (L1 > 0) {
         h.begin(h.ctx, "ntt_col_fwd");
-        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE, V>), dim3(ntiles, 2 * NP, nl), dim3(NTT_THREADS), col_lds, stream, L);
+        const int of = L.order & 3;
+        const dim3 g = of == COLF_OP_PRIME ? dim3(ntiles, 2 * NP, nl)
+                       : of == COLF_TILE_PRIME ? dim3(ntiles * NP, 2, nl) : dim3(ntiles * 2, NP, nl);
+        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE, V>), g, dim3(NTT_THREADS), col_lds, stream, L);
         h.end(h.ctx, "ntt_col_fwd", io + 16.0 * NP * n);
     }
     h.begin(h.ctx, "ntt_row");

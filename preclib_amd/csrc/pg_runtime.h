@@ -93,6 +93,8 @@ public:
         return (mode == MODE_WIDE && logn >= 0 && logn <= NTT_MAX_LOG) ? ntt_variant_tab[logn] : 0;
     }
     void load_plan_table();
+    // workgroup order of the NTT kernels (NttLaunch::order); $PGPU_NTT_ORDER overrides
+    int ntt_order = COLF_OP_PRIME | ROW_PRIME_REV;
     static constexpr size_t NWORDS = 4096;
     uint64_t *words = nullptr;    // device flags / small results of linear steps
 

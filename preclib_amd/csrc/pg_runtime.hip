@@ -138,6 +138,7 @@ int Context::init(int dev, size_t budget) {
     HIP_TRY(hipMalloc(&maxerr, 256));
     HIP_TRY(hipMalloc(&words, NWORDS * sizeof(uint64_t)));
     load_plan_table();
+    if (const char *e = std::getenv("PGPU_NTT_ORDER")) ntt_order = std::atoi(e) & 7;
     {
         const long double PI = 3.141592653589793238462643383279502884L;
         for (int k = 0; k < 8; ++k) {
@@ -543,6 +544,7 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     L.io_bytes = (double)(an + bn) * 16.0;
     L.logn = logn; L.mode = mode;
     L.variant = C.ntt_variant(logn, mode);
+    L.order = C.ntt_order;
     { int st = fill_ntt_tables(L); if (st) return st; }
     const int l1 = plan_l1(logn, mode == MODE_WIDE ? L.variant : 0);
     if (l1) L.one.spec = s.get<uint32_t>(2 * NP * n);
