@@ -25,6 +25,7 @@
 namespace pg {
 
 __device__ __forceinline__ int fp(int i) { return i + (i >> 5); }   // padded LDS index
+constexpr size_t fft_lds_words(size_t n) { return n + (n >> 5); }   // LDS slots for fp(0 .. n-1)
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -390,7 +391,7 @@ template <int LOGN>
 static hipError_t launch_fft_size(const FftLaunch &L, hipStream_t stream, const StageHook &h) {
     constexpr int L1 = fft_l1(LOGN), L2 = fft_l2(LOGN), LT = fft_lt(LOGN);
     const int ntiles = (1 << L2) >> LT;
-    const size_t col_lds = lds_words((size_t)1 << (L1 + LT)) * 16, row_lds = lds_words((size_t)1 << L2) * 16;
+    const size_t col_lds = fft_lds_words((size_t)1 << (L1 + LT)) * 16, row_lds = fft_lds_words((size_t)1 << L2) * 16;
     const int nl = L.leaves ? L.nleaves : 1;
     const double n = (double)(1u << LOGN) * nl, io = L.io_bytes;
     if constexpr (L1 > 0) {

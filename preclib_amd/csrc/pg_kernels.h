@@ -16,12 +16,9 @@ constexpr int ROW_MAX_ELEMS = 1 << 15;      // longest in-LDS row transform (N2)
 constexpr int COL_MAX_ELEMS = 1 << 14;      // largest column tile (N1 * T)
 constexpr int MAX_PASS = 6;
 
-// LDS words for n elements with one pad word per 32 (see lp() in pg_ntt.hip).
-#ifndef PG_LDS_PAD_LOG
-#define PG_LDS_PAD_LOG 5
-#endif
-constexpr int LDS_PAD_LOG = PG_LDS_PAD_LOG;   // one pad word after every 2^LDS_PAD_LOG words
-constexpr size_t lds_words(size_t n) { return n + (n >> LDS_PAD_LOG); }
+// LDS words for an n-element NTT tile: the layout permutes words within each
+// 64-word row (lp() in pg_ntt_impl.h), so it needs no padding.
+constexpr size_t lds_words(size_t n) { return n; }
 
 // ---- transform plan, shared by host and kernels ---------------------------
 // n = 2^logn = N1 * N2: rows of N2 = 2^plan_l2 points are transformed in LDS

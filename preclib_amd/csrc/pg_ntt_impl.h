@@ -34,9 +34,13 @@
 
 namespace pg {
 
-// LDS element i lives at word i + i / 2^LDS_PAD_LOG, so the strided accesses
-// of the radix passes fall in distinct banks.
-__device__ __forceinline__ int lp(int i) { return i + (i >> LDS_PAD_LOG); }
+// LDS element i lives at word i ^ (9 * ((i >> 6) & 7)): within each 64-word
+// row (one word per bank of gfx950's 64 banks) the bank is XORed with the row
+// number in both 3-bit halves.  Across the radix passes a wave touches 64
+// words of the same q-th input at strides 1, 8, 64 and above; the XOR sends
+// each of those patterns to 64 distinct banks (32-word padding leaves 2- to
+// 4-way conflicts in the passes of stride 8 and in the last pass).
+__device__ __forceinline__ int lp(int i) { return i ^ (9 * ((i >> 6) & 7)); }
 
 // Workgroup barrier for LDS traffic: waits for this wave's LDS operations and
 // the other waves, not for its outstanding global loads.  Every exchange
@@ -216,17 +220,18 @@ __device__ __forceinline__ uint32_t reduce32(uint32_t x, uint32_t red, uint32_t 
     return min(r, r - p);
 }
 
-// The digits of one limb, as residues mod the transform's prime.
+// The digits of one limb, as residues mod the transform's prime, into tile
+// elements i, i + 1, ...
 template <int MODE>
-__device__ __forceinline__ void put_digits(uint32_t *d, uint64_t v, uint32_t red, uint32_t p) {
+__device__ __forceinline__ void put_digits(uint32_t *lds, int i, uint64_t v, uint32_t red, uint32_t p) {
     if constexpr (MODE == MODE_REF) {
-        d[0] = (uint32_t)(v & 0xFFFF);
-        d[1] = (uint32_t)((v >> 16) & 0xFFFF);
-        d[2] = (uint32_t)((v >> 32) & 0xFFFF);
-        d[3] = (uint32_t)(v >> 48);
+        lds[lp(i)] = (uint32_t)(v & 0xFFFF);
+        lds[lp(i + 1)] = (uint32_t)((v >> 16) & 0xFFFF);
+        lds[lp(i + 2)] = (uint32_t)((v >> 32) & 0xFFFF);
+        lds[lp(i + 3)] = (uint32_t)(v >> 48);
     } else {
-        d[0] = reduce32((uint32_t)v, red, p);
-        d[1] = reduce32((uint32_t)(v >> 32), red, p);
+        lds[lp(i)] = reduce32((uint32_t)v, red, p);
+        lds[lp(i + 1)] = reduce32((uint32_t)(v >> 32), red, p);
     }
 }
 
@@ -274,8 +279,7 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
         for (int e = threadIdx.x; e < N1 * LPR; e += NTT_THREADS) {
             const int j1 = e / LPR, m = e % LPR;
             const size_t li = (((size_t)j1 << L2) + c0 + DPL * m) / DPL;
-            // DPL aligned words never straddle a pad
-            put_digits<MODE>(lds + lp(j1 * T + DPL * m), li < nl ? src[li] : 0, L.crt.red[pr], c.p);
+            put_digits<MODE>(lds, j1 * T + DPL * m, li < nl ? src[li] : 0, L.crt.red[pr], c.p);
         }
         lds_sync();
         lds_forward<L1, LT, NTT_THREADS, RBV>(lds, L.col_tw[pr], c);
@@ -314,14 +318,13 @@ __device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, cons
         const uint64_t *src = op ? D.b : D.a;
         const size_t nl = op ? D.bn : D.an;
         for (int m = threadIdx.x; m < N2 / DPL; m += ROW_THREADS)
-            put_digits<MODE>(lds + lp(DPL * m), (size_t)m < nl ? src[m] : 0, L.crt.red[pr], L.pk[pr].p);
+            put_digits<MODE>(lds, DPL * m, (size_t)m < nl ? src[m] : 0, L.crt.red[pr], L.pk[pr].p);
     } else {
         const uint4 *in = (const uint4 *)(D.spec + ((size_t)(op * mode_primes(MODE) + pr) << LOGN) +
                                           ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
             const uint4 v = in[m];
-            uint32_t *d = lds + lp(4 * m);
-            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+            lds[lp(4 * m)] = v.x; lds[lp(4 * m + 1)] = v.y; lds[lp(4 * m + 2)] = v.z; lds[lp(4 * m + 3)] = v.w;
         }
     }
 }
@@ -364,8 +367,8 @@ k_row(const NttLaunch L) {
         for (int k = 0; k < E4; ++k) {
             const int m = threadIdx.x + k * ROW_THREADS;
             if (m < N2 / 4) {
-                uint32_t *d = lds + lp(4 * m);
-                d[0] = pb[k].x; d[1] = pb[k].y; d[2] = pb[k].z; d[3] = pb[k].w;
+                lds[lp(4 * m)] = pb[k].x; lds[lp(4 * m + 1)] = pb[k].y;
+                lds[lp(4 * m + 2)] = pb[k].z; lds[lp(4 * m + 3)] = pb[k].w;
             }
         }
     } else {
@@ -382,8 +385,7 @@ k_row(const NttLaunch L) {
     lds_inverse<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_itw[pr], c);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
-        const uint32_t *d = lds + lp(4 * m);
-        out[m] = make_uint4(d[0], d[1], d[2], d[3]);
+        out[m] = make_uint4(lds[lp(4 * m)], lds[lp(4 * m + 1)], lds[lp(4 * m + 2)], lds[lp(4 * m + 3)]);
     }
 }
 
@@ -578,9 +580,8 @@ __device__ __forceinline__ void col_inv_ref(uint32_t *lds, const NttLaunch &L, c
             const int e = threadIdx.x + k * COLINV_THREADS;
             if (e < NL) {
                 const int rho = e / LPR, m = e % LPR;
-                const uint32_t *w = lds + lp(rho * T + 4 * m);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) acc128(slo[k], shi[k], w[t], 16 * t + 32 * half);
+                for (int t = 0; t < 4; ++t) acc128(slo[k], shi[k], lds[lp(rho * T + 4 * m + t)], 16 * t + 32 * half);
             }
         }
         lds_sync();
