@@ -120,22 +120,23 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
 }
 
 // Pass lists: forward runs passes 0..np-1 with shrinking block size; the
-// inverse runs them backwards (post-order recursion).
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB>
+// inverse runs them backwards (post-order recursion).  Passes I >= STOP are
+// left out (the row kernel runs its last pass in registers).
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP>
 __device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
-    if constexpr (LR > 0) {
+    if constexpr (LR > 0 && I < STOP) {
         pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c, tid);
         lds_sync();
-        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, tw, c, tid);
+        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lds, tw, c, tid);
     }
 }
 
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB>
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP>
 __device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
-    if constexpr (LR > 0) {
-        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB>(lds, itw, c, tid);
+    if constexpr (LR > 0 && I < STOP) {
+        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lds, itw, c, tid);
         pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c, tid);
         lds_sync();
     }
@@ -143,16 +144,16 @@ __device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw,
 
 // tid: the thread index (callers may pass an opaque copy of threadIdx.x so
 // that index arithmetic is recomputed per call rather than kept live).
-template <int LOGN, int LOGT, int NT, int RB>
+template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS>
 __device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c,
                                             int tid = threadIdx.x) {
-    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, tw, c, tid);
+    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP>(lds, tw, c, tid);
 }
 
-template <int LOGN, int LOGT, int NT, int RB>
+template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS>
 __device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c,
                                             int tid = threadIdx.x) {
-    inv_from<LOGN, LOGT, NT, LOGN, 0, RB>(lds, itw, c, tid);
+    inv_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP>(lds, itw, c, tid);
 }
 
 // Frequency index held at position rho after the forward passes of a 2^LOGN
@@ -333,12 +334,19 @@ template <int LOGN, int MODE, int V>
 __global__ void __launch_bounds__(row_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(row_waves(LOGN, V))))
 k_row(const NttLaunch L) {
     constexpr int L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
-    constexpr int E = (N2 + ROW_THREADS - 1) / ROW_THREADS;
+    constexpr int RBV = plan_rb(LOGN, V);
+    // The last forward pass (block size R_L, no twiddles) runs in registers:
+    // thread t holds the outputs of last-pass columns t + it * ROW_THREADS,
+    // positions col * R_L + bitrev(q), through the pointwise product and the
+    // inverse's first pass, which reads exactly those positions.
+    constexpr int NPS = split_np(L2, RBV);
+    constexpr int LRL = split_lr(L2, NPS - 1, RBV), RL = 1 << LRL;
+    constexpr int NCL = N2 >> LRL, IL = (NCL + ROW_THREADS - 1) / ROW_THREADS;
     extern __shared__ uint32_t lds[];
     const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
     const PrimeK &c = L.pk[pr];
-    uint32_t keep[E];
+    uint32_t keep[IL][RL];
 
     constexpr int E4 = (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS;
     // larger rows would spill; radix-8 rows at 64 VGPRs rely on occupancy instead
@@ -355,11 +363,15 @@ k_row(const NttLaunch L) {
         }
     }
     lds_sync();
-    lds_forward<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_tw[pr], c);
+    lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1>(lds, L.row_tw[pr], c);
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int i = threadIdx.x + k * ROW_THREADS;
-        if (i < N2) keep[k] = lds[lp(i)];
+    for (int it = 0; it < IL; ++it) {
+        const int col = threadIdx.x + it * ROW_THREADS;
+        if (col < NCL) {
+#pragma unroll
+            for (int q = 0; q < RL; ++q) keep[it][q] = lds[lp(col * RL + q)];
+            dft_dif<RL>(keep[it], c);
+        }
     }
     lds_sync();
     if constexpr (PF) {
@@ -375,18 +387,30 @@ k_row(const NttLaunch L) {
         load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
     }
     lds_sync();
-    lds_forward<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_tw[pr], c);
+    lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1>(lds, L.row_tw[pr], c);
+    // B's last pass, pointwise product and the inverse's first pass share the
+    // registers; every thread reads its inputs before any writes its outputs
+    // to the same positions, within one column
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-        const int i = threadIdx.x + k * ROW_THREADS;
-        if (i < N2) lds[lp(i)] = mont_mul(lds[lp(i)], keep[k], c.p, c.ninv);
+    for (int it = 0; it < IL; ++it) {
+        const int col = threadIdx.x + it * ROW_THREADS;
+        if (col < NCL) {
+            uint32_t x[RL];
+#pragma unroll
+            for (int q = 0; q < RL; ++q) x[q] = lds[lp(col * RL + q)];
+            dft_dif<RL>(x, c);
+#pragma unroll
+            for (int q = 0; q < RL; ++q) x[q] = mont_mul(x[q], keep[it][q], c.p, c.ninv);
+            dft_dit_inv<RL>(x, c);
+#pragma unroll
+            for (int q = 0; q < RL; ++q) lds[lp(col * RL + q)] = x[q];
+        }
     }
     lds_sync();
-    lds_inverse<L2, 0, ROW_THREADS, plan_rb(LOGN, V)>(lds, L.row_itw[pr], c);
+    lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1>(lds, L.row_itw[pr], c);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
-    for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
+    for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
         out[m] = make_uint4(lds[lp(4 * m)], lds[lp(4 * m + 1)], lds[lp(4 * m + 2)], lds[lp(4 * m + 3)]);
-    }
 }
 
 // ---- kernel 3: inverse column pass, CRT, limb sums ------------------------------
