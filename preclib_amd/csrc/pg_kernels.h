@@ -84,6 +84,12 @@ constexpr bool plan_distinct(int logn, int v) {
 constexpr int row_threads(int logn, int v = 0) {
     return (plan_l2(logn, v) >= 15 || (plan_rb(logn, v) == 3 && plan_l2(logn, v) >= 13)) ? 1024 : 512;
 }
+// Row kernel with both operands' rows in LDS (2 N2 words): for rows of up to
+// 2^13 points (64 KB, two workgroups per CU).
+#ifndef PG_ROW_DUAL
+#define PG_ROW_DUAL 1
+#endif
+constexpr bool row_dual(int logn, int v = 0) { return PG_ROW_DUAL && plan_l2(logn, v) <= 13; }
 // Minimum waves per SIMD asked of the compiler for the row kernel: 8 (64
 // VGPRs, two 1024-thread workgroups per CU) for radix-8 rows of 8 values per
 // thread; otherwise no request beyond what the workgroup size needs.

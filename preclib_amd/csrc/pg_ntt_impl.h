@@ -92,6 +92,42 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
     }
 }
 
+// pass_fwd over two tiles of the same shape (the row kernel's two operands):
+// each column's twiddles are loaded once and used for both.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+__device__ __forceinline__ void pass_fwd2(uint32_t *lda, uint32_t *ldb, const uint2 *__restrict__ tw, const PrimeK &c,
+                                          int tid) {
+    constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
+    constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
+    constexpr int S = M * T;
+#pragma unroll 1
+    for (int col = tid; col < NCOLS; col += NT) {
+        const int cc = col & (T - 1), r = col >> LOGT;
+        const int blk = r >> LOGM, j2 = r & (M - 1);
+        const int base = (((blk << LOGB) + j2) << LOGT) + cc;
+        uint2 w[R];
+        if constexpr (M > 1) {
+            const uint2 *t = tw + j2;
+#pragma unroll
+            for (int q = 1; q < R; ++q) w[q] = t[bitrev<R>(q) * M];
+        }
+#pragma unroll
+        for (int op = 0; op < 2; ++op) {
+            uint32_t *lds = op ? ldb : lda;
+            uint32_t x[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
+            dft_dif<R>(x, c);
+            if constexpr (M > 1) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) x[q] = mul_shoup(x[q], w[q].x, w[q].y, c.p);
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lp(base + bitrev<R>(q) * S)] = x[q];
+        }
+    }
+}
+
 template <int LR, int LOGN, int LOGT, int LOGB, int NT>
 __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict__ itw, const PrimeK &c, int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
@@ -129,6 +165,17 @@ __device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, 
         pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c, tid);
         lds_sync();
         fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lds, tw, c, tid);
+    }
+}
+
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP>
+__device__ __forceinline__ void fwd2_from(uint32_t *lda, uint32_t *ldb, const uint2 *const *tw, const PrimeK &c,
+                                          int tid) {
+    constexpr int LR = split_lr(LOGN, I, RB);
+    if constexpr (LR > 0 && I < STOP) {
+        pass_fwd2<LR, LOGN, LOGT, LOGB, NT>(lda, ldb, tw[I], c, tid);
+        lds_sync();
+        fwd2_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lda, ldb, tw, c, tid);
     }
 }
 
@@ -346,6 +393,41 @@ k_row(const NttLaunch L) {
     const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
     const PrimeK &c = L.pk[pr];
+    if constexpr (row_dual(LOGN, V)) {
+        // Both operands' rows in LDS (A at 0, B at N2): every forward pass
+        // transforms a column of each with one load of its twiddles, then B's
+        // last pass, the pointwise product and the inverse's first pass run in
+        // registers, and the inverse finishes in B's tile.
+        uint32_t *lda = lds, *ldb = lds + N2;
+        load_row<LOGN, MODE, V>(lda, L, D, 0, pr, row);
+        load_row<LOGN, MODE, V>(ldb, L, D, 1, pr, row);
+        lds_sync();
+        fwd2_from<L2, 0, ROW_THREADS, L2, 0, RBV, NPS - 1>(lda, ldb, L.row_tw[pr], c, threadIdx.x);
+#pragma unroll
+        for (int it = 0; it < IL; ++it) {
+            const int col = threadIdx.x + it * ROW_THREADS;
+            if (col < NCL) {
+                uint32_t xa[RL], x[RL];
+#pragma unroll
+                for (int q = 0; q < RL; ++q) xa[q] = lda[lp(col * RL + q)];
+                dft_dif<RL>(xa, c);
+#pragma unroll
+                for (int q = 0; q < RL; ++q) x[q] = ldb[lp(col * RL + q)];
+                dft_dif<RL>(x, c);
+#pragma unroll
+                for (int q = 0; q < RL; ++q) x[q] = mont_mul(x[q], xa[q], c.p, c.ninv);
+                dft_dit_inv<RL>(x, c);
+#pragma unroll
+                for (int q = 0; q < RL; ++q) ldb[lp(col * RL + q)] = x[q];
+            }
+        }
+        lds_sync();
+        lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1>(ldb, L.row_itw[pr], c);
+        uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
+        for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
+            out[m] = make_uint4(ldb[lp(4 * m)], ldb[lp(4 * m + 1)], ldb[lp(4 * m + 2)], ldb[lp(4 * m + 3)]);
+        return;
+    }
     uint32_t keep[IL][RL];
 
     constexpr int E4 = (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS;
@@ -648,7 +730,8 @@ template <int LOGN, int MODE, int V>
 static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const StageHook &h) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), NP = mode_primes(MODE);
     const int ntiles = (1 << L2) >> LT;
-    const size_t col_lds = lds_words((size_t)1 << (L1 + LT)) * 4, row_lds = lds_words((size_t)1 << L2) * 4;
+    const size_t col_lds = lds_words((size_t)1 << (L1 + LT)) * 4,
+                 row_lds = lds_words((size_t)1 << L2) * 4 * (row_dual(LOGN, V) ? 2 : 1);
     const int nl = L.leaves ? L.nleaves : 1;
     // byte-model shares use the reference's P = 2 (see bench.py)
     const double n = (double)(1u << LOGN) * nl, io = L.io_bytes;
