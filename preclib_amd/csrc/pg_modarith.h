@@ -58,11 +58,18 @@ __device__ __forceinline__ constexpr int bitrev(int q) {
 
 // R-point forward DFT in registers, Gentleman-Sande DIF: natural order in,
 // x[q] = X[bitrev(q)] out.  Twiddles omega_R^k = omega_32^(k * 32 / R).
-template <int R>
+// HALFZ: x[R/2 .. R) are zero on entry, so the first stage is only the
+// twiddle products of the lower half.
+template <int R, bool HALFZ = false>
 __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
     const uint32_t p = c.p;
+    if constexpr (HALFZ) {
 #pragma unroll
-    for (int h = R / 2; h >= 1; h >>= 1) {
+        for (int j = 0; j < R / 2; ++j)
+            x[j + R / 2] = j == 0 ? x[0] : mul_shoup(x[j], c.w32[j * (32 / R)], c.w32q[j * (32 / R)], p);
+    }
+#pragma unroll
+    for (int h = HALFZ ? R / 4 : R / 2; h >= 1; h >>= 1) {
 #pragma unroll
         for (int b = 0; b < R; b += 2 * h) {
 #pragma unroll

@@ -65,7 +65,9 @@ __device__ __forceinline__ int fresh_tid() {
 
 // ---- in-LDS radix passes --------------------------------------------------
 
-template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+// HALFZ (first pass only): inputs q >= R/2, the upper half of the
+// transform, are zero and are neither read nor added.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool HALFZ = false>
 __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict__ tw, const PrimeK &c, int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
     constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
@@ -77,8 +79,8 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
         const int base = (((blk << LOGB) + j2) << LOGT) + cc;
         uint32_t x[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
-        dft_dif<R>(x, c);
+        for (int q = 0; q < (HALFZ ? R / 2 : R); ++q) x[q] = lds[lp(base + q * S)];
+        dft_dif<R, HALFZ>(x, c);
         if constexpr (M > 1) {
             const uint2 *t = tw + j2;
 #pragma unroll
@@ -321,16 +323,29 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
     const uint64_t *src = op ? D.b : D.a;
     const size_t nl = op ? D.bn : D.an;
     const uint32_t C = c0 + (threadIdx.x & (T - 1));
+    // An operand of at most n/2 digits leaves rows N1/2 .. N1 of every column
+    // zero: they are not staged and the first pass skips them.
+    constexpr int LR0 = split_lr(L1, 0, RBV);
+    const bool half = L1 > 0 && LR0 > 0 && (size_t)nl * DPL <= ((size_t)1 << (LOGN - 1));
     {
         const PrimeK &c = L.pk[pr];
         // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T)
-        for (int e = threadIdx.x; e < N1 * LPR; e += NTT_THREADS) {
+        const int ne = (half ? N1 / 2 : N1) * LPR;
+        for (int e = threadIdx.x; e < ne; e += NTT_THREADS) {
             const int j1 = e / LPR, m = e % LPR;
             const size_t li = (((size_t)j1 << L2) + c0 + DPL * m) / DPL;
             put_digits<MODE>(lds, j1 * T + DPL * m, li < nl ? src[li] : 0, L.crt.red[pr], c.p);
         }
         lds_sync();
-        lds_forward<L1, LT, NTT_THREADS, RBV>(lds, L.col_tw[pr], c);
+        if constexpr (LR0 > 0) {
+            if (half) {
+                pass_fwd<LR0, L1, LT, L1, NTT_THREADS, true>(lds, L.col_tw[pr][0], c, threadIdx.x);
+                lds_sync();
+                fwd_from<L1, LT, NTT_THREADS, L1 - LR0, 1, RBV, MAX_PASS>(lds, L.col_tw[pr], c, threadIdx.x);
+            } else {
+                lds_forward<L1, LT, NTT_THREADS, RBV>(lds, L.col_tw[pr], c);
+            }
+        }
         uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
         if constexpr (TW::STEP) {
             const uint32_t f0 = freq_of_pos<L1, RBV>(threadIdx.x >> LT);
