@@ -363,14 +363,20 @@ hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream) {
     if (!C.nchunks) return hipSuccess;
     if (!C.agg || !C.cin) return hipErrorInvalidValue;
     const unsigned nch = (unsigned)C.nchunks, nfix = (unsigned)((C.nchunks + CARRY_WAVES - 1) / CARRY_WAVES);
+    // every chunk a node start: all carry-ins are exact in the first pass
+    const bool one_pass = C.nnodes == C.nchunks;
     if (!C.nodes && C.one.kind == NODE_PLAIN && C.one.parts == 1) {
         hipLaunchKernelGGL(k_carry_spec<true>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
-        hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
-        hipLaunchKernelGGL(k_carry_fix<true>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+        if (!one_pass) {
+            hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
+            hipLaunchKernelGGL(k_carry_fix<true>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+        }
     } else {
         hipLaunchKernelGGL(k_carry_spec<false>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
-        hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
-        hipLaunchKernelGGL(k_carry_fix<false>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+        if (!one_pass) {
+            hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
+            hipLaunchKernelGGL(k_carry_fix<false>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+        }
     }
     return hipGetLastError();
 }

@@ -96,6 +96,7 @@ int lin_run(const CarryNode *nodes, size_t n, const char *stage) {
     DBuf flags((2 * nch + 7) / 8);
     if (!flags.d) return set_error(PG_ENOMEM, "carry scratch");
     cl.nchunks = nch;
+    cl.nnodes = n;
     cl.agg = (uint8_t *)flags.d;
     cl.cin = (uint8_t *)flags.d + nch;
     hipEvent_t e0 = nullptr;

@@ -655,8 +655,11 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
     };
     std::vector<DNode> nodes{{x, xn, xb, 0}};
     std::vector<DBuf> keep;
-    DBuf flags;
-    std::vector<size_t> check_flags;   // indices of words that must read 1 (ok) / 0
+    struct Check {
+        const uint64_t *flags;   // 4 words per division: r >= 0 (1), -, r2 >= W (0), -
+        size_t n;
+    };
+    std::vector<Check> checks;
     for (int i = top - 1; i >= leaf_log; --i) {
         const size_t nw = wl[i], vb = vl[i], sb = 2 * wb[i];
         std::vector<DNode> next;
@@ -734,10 +737,9 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
         if ((st = lin_run(lr.data(), nj, "decimal")) || (st = lin_run(lt.data(), nj, "decimal")) ||
             (st = lin_run(lc.data(), 2 * nj, "decimal")) || (st = lin_run(lv.data(), nj, "decimal")))
             return st;
-        std::vector<uint64_t> f(4 * nj);
-        if ((st = read_words(F, f.size(), f.data()))) return st;
-        for (size_t j = 0; j < nj; ++j)
-            if (f[4 * j] != 1 || f[4 * j + 2] != 0) return set_error(PG_EINTERNAL, "decimal split check");
+        // the split checks are read once, after the last level: the host plans
+        // the next level from bit bounds alone, so levels queue back to back
+        checks.push_back({F, nj});
         keep.push_back(std::move(fl));
         nodes.swap(next);
     }
@@ -762,7 +764,15 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
     HIP_TRY(hipGetLastError());
     std::string s(dcap, '0');
     HIP_TRY(hipMemcpyAsync(&s[0], chars.d, dcap, hipMemcpyDeviceToHost, C.stream));
+    std::vector<std::vector<uint64_t>> fh(checks.size());
+    for (size_t c = 0; c < checks.size(); ++c) {
+        fh[c].resize(4 * checks[c].n);
+        HIP_TRY(hipMemcpyAsync(fh[c].data(), checks[c].flags, fh[c].size() * 8, hipMemcpyDeviceToHost, C.stream));
+    }
     if ((st = C.sync())) return st;
+    for (const auto &f : fh)
+        for (size_t j = 0; j < f.size(); j += 4)
+            if (f[j] != 1 || f[j + 2] != 0) return set_error(PG_EINTERNAL, "decimal split check");
     const size_t first = s.find_first_not_of('0');
     out = first == std::string::npos ? std::string() : s.substr(first);
     return PG_OK;

@@ -438,6 +438,7 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
     cl.one.out = out;
     cl.one.out_len = out_len;
     cl.nchunks = (std::max(m, out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
+    cl.nnodes = 1;
     cl.agg = s.get<uint8_t>(cl.nchunks);
     cl.cin = s.get<uint8_t>(cl.nchunks);
     if (!cl.agg || !cl.cin) return set_error(PG_ENOMEM, "carry scratch");
@@ -505,6 +506,7 @@ int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dno
     cl.nodes = dnodes;
     cl.chunk_node = dchunk;
     cl.nchunks = nchunks;
+    cl.nnodes = nodes.size();
     cl.agg = agg;
     cl.cin = cin;
     double bytes = 0;
