@@ -59,7 +59,7 @@ private:
         size_t first_chunk = 0, first_node = 0, count = 0, nchunks = 0;
         const CarryNode *dnodes = nullptr;
         const uint32_t *dchunk = nullptr;
-        uint8_t *agg = nullptr, *cin = nullptr;
+        uint8_t *agg = nullptr;
         std::vector<CarryNode> host;
     };
     SumRef new_sums(size_t len);

@@ -25,7 +25,7 @@
 namespace pg {
 
 int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dnodes, const uint32_t *dchunk,
-                       size_t nchunks, uint8_t *agg, uint8_t *cin);
+                       size_t nchunks, uint8_t *agg);
 
 namespace {
 
@@ -232,7 +232,6 @@ int Batch::build(const pg_job *jobs, size_t n) {
     for (auto &nd : nodes) nchunks += (std::max(nd.m, nd.to_sums ? nd.out_sums.len : nd.out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
     const size_t d_chunkmap = al.take(nchunks * 4);
     const size_t d_agg = al.take(nchunks);
-    const size_t d_cin = al.take(nchunks);
     arena_bytes = al.bytes;
     arena = (char *)C.pool.get(arena_bytes);
     alloc = nullptr;
@@ -361,7 +360,6 @@ int Batch::build(const pg_job *jobs, size_t n) {
         lv.dnodes = (const CarryNode *)(P(d_nodes) + nidx * sizeof(CarryNode));
         lv.dchunk = (const uint32_t *)(P(d_chunkmap) + lv.first_chunk * 4);
         lv.agg = (uint8_t *)P(d_agg) + lv.first_chunk;
-        lv.cin = (uint8_t *)P(d_cin) + lv.first_chunk;
         lv.host = lvnodes;
         nidx += lvnodes.size();
         if (lv.count) levels.push_back(lv);
@@ -418,7 +416,7 @@ int Batch::enqueue() {
             return set_error(PG_EDEVICE, "pg_batch: small-product launch failed");
     }
     for (auto &lv : levels) {
-        int st = launch_carry_nodes(lv.host, lv.dnodes, lv.dchunk, lv.nchunks, lv.agg, lv.cin);
+        int st = launch_carry_nodes(lv.host, lv.dnodes, lv.dchunk, lv.nchunks, lv.agg);
         if (st) return st;
     }
     return PG_OK;

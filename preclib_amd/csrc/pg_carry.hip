@@ -13,8 +13,8 @@
 // the carry out of the group is the overflow of that 64-bit sum.  Each wave
 // walks CARRY_ITERS such groups (coalesced loads, the wave carry chained in
 // registers); chunks of CARRY_CHUNK limbs take their carry-in from the limbs
-// just below them and are checked and, where needed, fixed up by a
-// one-workgroup scan of the chunk aggregates (see k_carry_spec).
+// just below them and are, where that window propagates throughout, fixed
+// up from the chunk aggregates (see k_carry_spec).
 // Linear nodes reuse the same scan for device additions, subtractions,
 // shifts and small-scalar multiples (the add/sub/shift/mul_u32 steps that
 // div_newton.cpp, sqrt.cpp and pi_chudnovsky.cpp run between products).
@@ -134,8 +134,7 @@ __device__ __forceinline__ Agg compose(Agg lo, Agg hi) {   // lo first, then hi
     return Agg{hi.g | (hi.p & lo.g), hi.p & lo.p};
 }
 
-// One workgroup: exclusive scan of chunk aggregates into chunk carry-ins.
-// The first chunk of every node starts with carry 0 (a "kill" before it).
+// The first chunk of every node starts with its node's carry-in.
 __device__ __forceinline__ bool node_start(const CarryLaunch &C, size_t i) {
     return C.nodes ? C.nodes[C.chunk_node[i]].chunk0 == i : i == 0;
 }
@@ -145,39 +144,6 @@ __device__ __forceinline__ uint32_t node_cin(const CarryLaunch &C, size_t i) {
     return (N.kind == NODE_LIN && N.cin && lin_cond(N)) ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
-    __shared__ Agg waves[16];
-    __shared__ uint32_t wave_in[16];
-    const size_t per = (C.nchunks + 1023) / 1024;
-    const size_t lo = (size_t)threadIdx.x * per;
-    Agg a{0, 1};
-    for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
-        const uint32_t v = C.agg[i];
-        if (node_start(C, i)) a = Agg{node_cin(C, i), 0};
-        a = compose(a, Agg{v & 1, (v >> 1) & 1});
-    }
-    // thread-level (G, P) pairs -> carries via the wave sum trick
-    const Group G{__ballot(a.g | a.p), __ballot(a.g)};
-    if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = group_agg(G);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t x = 0;
-        for (int i = 0; i < 16; ++i) {
-            wave_in[i] = x;
-            x = waves[i].g | (waves[i].p & x);
-        }
-    }
-    __syncthreads();
-    const uint64_t sum = G.X + G.Y + wave_in[threadIdx.x >> 6];
-    uint32_t x = (uint32_t)(((sum ^ G.X ^ G.Y) >> (threadIdx.x & 63)) & 1);
-    for (size_t i = lo; i < lo + per && i < C.nchunks; ++i) {
-        if (node_start(C, i)) x = node_cin(C, i);
-        C.cin[i] = (uint8_t)x;
-        const uint32_t v = C.agg[i];
-        x = (v & 1) | ((v >> 1) & x);
-    }
-}
-
 // ---- speculative form: carry-in from the limbs just below each chunk --------
 //
 // The carry into a chunk is the carry out of the 64 limbs below it unless all
@@ -185,10 +151,11 @@ __global__ void __launch_bounds__(1024) k_carry_scan(const CarryLaunch C) {
 // k_carry_spec computes that window with the chunk itself and applies the
 // carry it gives, so every chunk is written in one streaming pass with no
 // dependence on other workgroups.  A chunk whose window propagates
-// throughout is written with carry-in 0 and marked unresolved; the scan then
-// gives every chunk its true carry-in from the chunk aggregates, and
-// k_carry_fix adds the missing 1 to the unresolved chunks whose carry-in is
-// 1 (a +1 that ripples through the chunk's propagating prefix).
+// throughout is written with carry-in 0 and marked unresolved; k_carry_fix
+// then finds each unresolved chunk's true carry-in by walking back over the
+// chunk aggregates to the nearest chunk whose carry-out is known (generate,
+// kill, or resolved with its carry-in), and adds the missing 1 where it is 1
+// (a +1 that ripples through the chunk's propagating prefix).
 //
 // agg[i] = g | p << 1 | used carry-in << 2 | resolved << 3.
 
@@ -329,17 +296,40 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch 
     }
 }
 
-// One wave per chunk: an unresolved chunk whose true carry-in (from the scan)
-// is 1 was written with 0; add 1 at its lowest limb, rippling through the
-// limbs that propagate (written as 2^64 - 1, now 0) into the first that
-// does not (written as t, now t + 1).  Limbs past out_len are not stored
-// but still carry the ripple; limb m is the flag word.
+// True carry-in of an unresolved chunk, by one wave: the carry out of the
+// nearest predecessor whose carry-out does not depend on its own carry-in
+// (it generates, it kills, or it was resolved and so used its exact carry-in)
+// passes unchanged through the propagating chunks between.  The walk ends at
+// the latest at the first chunk of the node, which is always resolved.
+__device__ uint32_t fix_carry_in(const CarryLaunch &C, size_t chunk) {
+    const int lane = threadIdx.x & 63;
+    long top = (long)chunk - 1;
+    for (;;) {
+        const long i = top - lane;
+        uint32_t d = 1, val = 0;
+        if (i >= 0) {
+            const uint32_t a = C.agg[i];
+            const uint32_t g = a & 1, p = (a >> 1) & 1, cu = (a >> 2) & 1, res = (a >> 3) & 1;
+            d = g | (p ^ 1) | res;
+            val = g ? 1u : (p ? cu : 0u);
+        }
+        const uint64_t dec = __ballot(d);
+        if (dec) return (uint32_t)__shfl((int)val, __ffsll((unsigned long long)dec) - 1);
+        top -= 64;
+    }
+}
+
+// One wave per chunk: an unresolved chunk whose true carry-in is 1 was
+// written with 0; add 1 at its lowest limb, rippling through the limbs that
+// propagate (written as 2^64 - 1, now 0) into the first that does not
+// (written as t, now t + 1).  Limbs past out_len are not stored but still
+// carry the ripple; limb m is the flag word.
 template <bool PLAIN>
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C) {
     const size_t chunk = (size_t)blockIdx.x * CARRY_WAVES + (threadIdx.x >> 6);
     if (chunk >= C.nchunks) return;
     const uint32_t v = C.agg[chunk];
-    if ((v & 8) || !C.cin[chunk]) return;
+    if ((v & 8) || !fix_carry_in(C, chunk)) return;
     const CarryNode &N = PLAIN ? C.one : node_of(C, chunk);
     const int lane = threadIdx.x & 63;
     const long cbase = (long)(chunk - N.chunk0) * CARRY_CHUNK;
@@ -361,22 +351,18 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C
 
 hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream) {
     if (!C.nchunks) return hipSuccess;
-    if (!C.agg || !C.cin) return hipErrorInvalidValue;
+    if (!C.agg) return hipErrorInvalidValue;
     const unsigned nch = (unsigned)C.nchunks, nfix = (unsigned)((C.nchunks + CARRY_WAVES - 1) / CARRY_WAVES);
     // every chunk a node start: all carry-ins are exact in the first pass
     const bool one_pass = C.nnodes == C.nchunks;
     if (!C.nodes && C.one.kind == NODE_PLAIN && C.one.parts == 1) {
         hipLaunchKernelGGL(k_carry_spec<true>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
-        if (!one_pass) {
-            hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
+        if (!one_pass)
             hipLaunchKernelGGL(k_carry_fix<true>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
-        }
     } else {
         hipLaunchKernelGGL(k_carry_spec<false>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
-        if (!one_pass) {
-            hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, stream, C);
+        if (!one_pass)
             hipLaunchKernelGGL(k_carry_fix<false>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
-        }
     }
     return hipGetLastError();
 }

@@ -93,12 +93,11 @@ int lin_run(const CarryNode *nodes, size_t n, const char *stage) {
         cl.chunk_node = (const uint32_t *)((const char *)desc.d + nb);
         C.hold(std::move(img));
     }
-    DBuf flags((2 * nch + 7) / 8);
+    DBuf flags((nch + 7) / 8);
     if (!flags.d) return set_error(PG_ENOMEM, "carry scratch");
     cl.nchunks = nch;
     cl.nnodes = n;
     cl.agg = (uint8_t *)flags.d;
-    cl.cin = (uint8_t *)flags.d + nch;
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
     HIP_TRY(launch_carry_kernels(cl, C.stream));

@@ -284,15 +284,13 @@ struct CarryLaunch {
     const uint32_t *chunk_node; // node of every chunk
     size_t nchunks;
     uint8_t *agg;               // per-chunk G | P << 1 | used carry-in << 2 | resolved << 3
-    uint8_t *cin;               // per-chunk true carry-in (from the scan)
     size_t nnodes;              // nodes of the launch (1 for `one`; 0: not stated)
 };
 
-__global__ void k_carry_scan(const CarryLaunch C);
-// Enqueues the carry of C on `stream`: the speculative pass, the scan of
-// chunk aggregates and the fix-up pass (pg_carry.hip).  When every chunk is
-// the first of its node (C.nchunks == C.nnodes), every carry-in is known in
-// the first pass and the other two are not launched.
+// Enqueues the carry of C on `stream`: the speculative pass and the fix-up
+// pass (pg_carry.hip).  When every chunk is the first of its node
+// (C.nchunks == C.nnodes), every carry-in is known in the first pass and the
+// fix-up is not launched.
 hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream);
 
 }  // namespace pg

@@ -440,8 +440,7 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
     cl.nchunks = (std::max(m, out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
     cl.nnodes = 1;
     cl.agg = s.get<uint8_t>(cl.nchunks);
-    cl.cin = s.get<uint8_t>(cl.nchunks);
-    if (!cl.agg || !cl.cin) return set_error(PG_ENOMEM, "carry scratch");
+    if (!cl.agg) return set_error(PG_ENOMEM, "carry scratch");
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
     HIP_TRY(launch_carry_kernels(cl, C.stream));
@@ -500,7 +499,7 @@ int fill_fft_tables(FftLaunch &L) {
 }
 
 int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dnodes, const uint32_t *dchunk,
-                       size_t nchunks, uint8_t *agg, uint8_t *cin) {
+                       size_t nchunks, uint8_t *agg) {
     Context &C = ctx();
     CarryLaunch cl{};
     cl.nodes = dnodes;
@@ -508,7 +507,6 @@ int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dno
     cl.nchunks = nchunks;
     cl.nnodes = nodes.size();
     cl.agg = agg;
-    cl.cin = cin;
     double bytes = 0;
     for (auto &n : nodes) bytes += (double)n.m * 16.0 * (n.kind == NODE_BLOCKED ? 2 : 1) + (double)n.out_len * 8.0;
     hipEvent_t e0 = nullptr;
