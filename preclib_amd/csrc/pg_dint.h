@@ -52,6 +52,10 @@ int bit_length(const uint64_t *a, size_t trimmed, size_t *bits);
 int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, size_t bits);
 // precn_reciprocal_newton (div_newton.cpp:70-87): exact floor(2^bits / b).
 int recip_exact(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits);
+// From an estimate x (xl limbs) within a few units of floor(2^bits / b), the
+// exact floor: r = 2^bits - b x, then the unit corrections of
+// div_newton.cpp:78-85.
+int recip_finish(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits);
 // div_mulinv_impl (div_newton.cpp:89-129): q = floor(a / b), r = a mod b.
 // a, b trimmed; q gets ql trimmed limbs, r (if non-null) rl trimmed limbs.
 int div_mulinv(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size_t an, const uint64_t *b,

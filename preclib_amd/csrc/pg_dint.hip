@@ -332,6 +332,11 @@ int recip_exact(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits) 
     int st = bit_length(b, bn, &nb);
     if (st) return st;
     if ((st = recip_approx(x, xl, b, bn, nb, bits))) return st;
+    return recip_finish(x, xl, b, bn, bits);
+}
+
+int recip_finish(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits) {
+    int st;
     if (!xl) return PG_OK;
     DBuf p(bn + xl);
     if (!p.d) return set_error(PG_ENOMEM, "reciprocal");

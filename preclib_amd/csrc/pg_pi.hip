@@ -646,8 +646,28 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
             if ((st = dmul(W[i].d, W[i - 1].d, wl[i - 1], W[i - 1].d, wl[i - 1]))) return st;
             if ((st = top_len(W[i].d, 2 * wl[i - 1], &wl[i])) || (st = bit_length(W[i].d, wl[i], &wb[i]))) return st;
         }
-        for (int i = leaf_log; i < top; ++i)
-            if ((st = recip_exact(V[i], vl[i], W[i].d, wl[i], 2 * wb[i]))) return st;
+        // V_top-1 by Newton's iteration; below it, since W_(i+1) = W_i^2,
+        //   2^(2 b_i) / W_i = W_i 2^(2 b_i) / W_(i+1),
+        // so floor(W_i V_(i+1) / 2^(2 b_(i+1) - 2 b_i)) is V_i or V_i - 1
+        // (V_(i+1) is within 1 of 2^(2 b_(i+1)) / W_(i+1), and W_i is below
+        // 2^(2 b_(i+1) - 2 b_i)); recip_finish makes it exact.
+        for (int i = top - 1; i >= leaf_log; --i) {
+            if (i == top - 1) {
+                if ((st = recip_exact(V[i], vl[i], W[i].d, wl[i], 2 * wb[i]))) return st;
+                continue;
+            }
+            const size_t pl = wl[i] + vl[i + 1], sh = 2 * wb[i + 1] - 2 * wb[i];
+            DBuf P(pl);
+            if (!P.d) return set_error(PG_ENOMEM, "powers");
+            if ((st = dmul(P.d, W[i].d, wl[i], V[i + 1].d, vl[i + 1]))) return st;
+            vl[i] = limbs_for_bits(wb[i] + 2) + 1;
+            if (!V[i].reset(vl[i])) return set_error(PG_ENOMEM, "powers");
+            CarryNode c = lin_node(V[i].d, vl[i]);
+            c.x = P.d;
+            c.xn = pl;
+            c.xsh = (int64_t)sh;
+            if ((st = lin_one(c, "decimal")) || (st = recip_finish(V[i], vl[i], W[i].d, wl[i], 2 * wb[i]))) return st;
+        }
     }
     struct DNode {
         const uint64_t *p;
