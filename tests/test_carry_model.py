@@ -1,0 +1,114 @@
+"""Model of the GPU carry resolution (pg_carry.hip: k_carry_spec, k_carry_fix)
+checked against Python integers on adversarial limb sums.
+
+Limb sums S_j = lo_j + 2^64 hi_j (hi_j < 2^60) are normalized chunk by chunk.
+A chunk takes its carry-in from the 64 limbs just below it (the highest one
+that does not propagate decides); where all 64 propagate it is written with
+carry-in 0 and marked unresolved.  The fix-up walks back over the chunk
+aggregates to the nearest chunk whose carry-out is known and adds the +1
+ripple where the true carry-in is 1.  The chunk here is 256 limbs (the
+kernel's is 2048), so short inputs cross many chunk boundaries.
+"""
+import random
+
+M64 = (1 << 64) - 1
+CHUNK = 256
+WINDOW = 64
+
+
+def normalize(slo, shi, out_len):
+    m = len(slo)
+    nch = (max(m, out_len) + CHUNK - 1) // CHUNK
+
+    def tgp(j):
+        hprev = shi[j - 1] if 1 <= j <= m else 0
+        if j < m:
+            t = (slo[j] + hprev) & M64
+            return t, int(t < slo[j]), int(t == M64)
+        return (hprev if j == m else 0), 0, 0
+
+    out = [0] * out_len
+    agg = []
+    for c in range(nch):
+        base = c * CHUNK
+        if c == 0:
+            cin, resolved = 0, 1
+        else:
+            win = [tgp(base - WINDOW + k) for k in range(WINDOW)]
+            stops = [k for k in range(WINDOW) if not win[k][2]]
+            cin, resolved = (win[max(stops)][1], 1) if stops else (0, 0)
+        x, g_all, p_all = cin, 0, 1
+        for j in range(base, base + CHUNK):
+            t, g, p = tgp(j)
+            if j < out_len:
+                out[j] = (t + x) & M64
+            x = g | (p & x)
+            g_all, p_all = g | (p & g_all), p & p_all
+        agg.append((g_all, p_all, cin, resolved))
+
+    def carry_in(c):
+        i = c - 1
+        while True:
+            g, p, used, res = agg[i]
+            if g:
+                return 1
+            if not p:
+                return 0
+            if res:
+                return used
+            i -= 1
+
+    for c in range(nch):
+        if agg[c][3] or not carry_in(c):
+            continue
+        for j in range(c * CHUNK, (c + 1) * CHUNK):
+            t, _, p = tgp(j)
+            if j < out_len:
+                out[j] = (t + 1) & M64
+            if not p:
+                break
+    return out
+
+
+def check(slo, shi, out_len):
+    v = sum((slo[j] + (shi[j] << 64)) << (64 * j) for j in range(len(slo)))
+    assert normalize(slo, shi, out_len) == [(v >> (64 * j)) & M64 for j in range(out_len)]
+
+
+def test_uniform_sums():
+    rng = random.Random(1)
+    for m in (1, 255, 256, 257, 1000):
+        slo = [rng.getrandbits(64) for _ in range(m)]
+        shi = [rng.getrandbits(50) for _ in range(m)]
+        check(slo, shi, m + 1)
+
+
+def test_long_propagate_runs():
+    # limbs that propagate (t = 2^64 - 1) for whole chunks, ended by a
+    # generate or a kill at chosen offsets around chunk and window edges
+    for end in (0, 1, 63, 64, 65, 191, 192, 255, 256, 257, 700):
+        m = 1100
+        slo, shi = [M64] * m, [0] * m
+        slo[end] = 0
+        shi[end] = 1      # limb end + 1 generates
+        check(slo, shi, m + 1)
+        slo, shi = [M64] * m, [0] * m
+        slo[end] = 5      # limb end kills
+        check(slo, shi, m)
+
+
+def test_all_ones_square_pattern():
+    # the limb pattern of (2^(64n) - 1)^2 carried from sums of one word each
+    n = 600
+    v = ((1 << (64 * n)) - 1) ** 2
+    slo = [(v >> (64 * j)) & M64 for j in range(2 * n)]
+    check(slo, [0] * (2 * n), 2 * n)
+
+
+def test_random_mixtures():
+    rng = random.Random(7)
+    for _ in range(20):
+        m = rng.choice([300, 513, 900])
+        slo = [M64 if rng.random() < 0.97 else rng.getrandbits(64) for _ in range(m)]
+        shi = [1 if rng.random() < 0.01 else 0 for _ in range(m)]
+        check(slo, shi, m + rng.choice([0, 1]))
