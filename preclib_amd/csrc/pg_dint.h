@@ -49,6 +49,11 @@ int bit_length(const uint64_t *a, size_t trimmed, size_t *bits);
 
 // newton64_reciprocal_approx (div_newton.cpp:36-68): x <= floor(2^bits / b)
 // within a few units; b trimmed (bn limbs, nb bits).  x gets xl limbs.
+// The seed floor(2^(keep + known) / top_ceil(b)) is the reference's, with
+// known = NEWTON_SEED_BITS instead of 64 (the reference is limited to one
+// machine word there): a host long division of about 0.3 ms that saves five
+// precision doublings.
+constexpr size_t NEWTON_SEED_BITS = 2048;
 int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, size_t bits);
 // precn_reciprocal_newton (div_newton.cpp:70-87): exact floor(2^bits / b).
 int recip_exact(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits);

@@ -207,7 +207,7 @@ int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, s
     xl = 0;
     if (!bn || !nb || bits < nb) return PG_OK;   // div_newton.cpp:37
     const size_t target = bits - nb;
-    size_t known = std::min<size_t>(target, 64);
+    size_t known = std::min<size_t>(target, NEWTON_SEED_BITS);
     const size_t keep = std::min(nb, known + 128);
     // bt = newton64_top_ceil(b, nb, keep) (div_newton.cpp:26-34) from the top limbs
     std::vector<uint64_t> bt;

@@ -128,10 +128,10 @@ int Server::admit(pg_request *u, std::unique_ptr<Req> &out) {
         q.stage = S_MUL;
         return PG_OK;
     }
-    // reciprocal seed (div_newton.cpp:41-47) from the host copy of b
     q.scale = q.na + 128;
     q.target = q.scale - q.nb;
-    q.known = std::min<size_t>(q.target, 64);
+    // reciprocal seed (div_newton.cpp:41-47) from the host copy of b
+    q.known = std::min<size_t>(q.target, NEWTON_SEED_BITS);
     const size_t keep = std::min(q.nb, q.known + 128), drop = q.nb - keep, l0 = drop / 64;
     const int r = (int)(drop % 64);
     std::vector<uint64_t> bt(q.bn - l0);
