@@ -357,7 +357,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=30,
+                    help="untimed steps before the timed ones (the GPU clocks need ~30 ms to ramp up)")
     ap.add_argument("--logn", type=int, default=22, help="operand size 2^logn limbs")
     ap.add_argument("--dist", default="uniform")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -394,10 +395,8 @@ def main():
     n = transform_len(a, b)
     da, db = pgpu.DevBuf.of(a), pgpu.DevBuf.of(b)
     r = pgpu.DevBuf(2 * N)
-    for _ in range(args.warmup):
-        pgpu.mul_dev(r, da, N, db, N, want_len=False)
-    pgpu.sync()
-    # correctness of the measured configuration (outside the timed region)
+    # correctness of the measured configuration (outside the timed region,
+    # before the warm-up, so that the timed steps follow the warm-up directly)
     rl, _ = pgpu.mul_dev(r, da, N, db, N)
     got = r.download(rl)
     check = None
@@ -410,14 +409,23 @@ def main():
         print("bench.py: product differs from the golden digest", file=sys.stderr)
         sys.exit(1)
 
-    pgpu.timing(True)
-    pgpu.timing_reset()
+    for _ in range(args.warmup):
+        pgpu.mul_dev(r, da, N, db, N, want_len=False)
+    # timed region: K products back to back, no per-stage events between the
+    # kernels
     pgpu.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pgpu.mul_dev(r, da, N, db, N, want_len=False)
     pgpu.sync()
     t1 = time.perf_counter()
+    # per-kernel launch times (HIP events on the library's stream) from a
+    # second pass of the same K products
+    pgpu.timing(True)
+    pgpu.timing_reset()
+    for _ in range(args.steps):
+        pgpu.mul_dev(r, da, N, db, N, want_len=False)
+    pgpu.sync()
     stages = pgpu.timing_read()
     pgpu.timing(False)
 
