@@ -356,7 +356,8 @@ def bench_serve(args, pgpu, workloads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 100 for the single product, 20 otherwise)")
     ap.add_argument("--warmup", type=int, default=30,
                     help="untimed steps before the timed ones (the GPU clocks need ~30 ms to ramp up)")
     ap.add_argument("--logn", type=int, default=22, help="operand size 2^logn limbs")
@@ -370,6 +371,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.workload == "single" else 20
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the product path "
               "shards as described in DESIGN.md, but no multi-GPU run is part of this build)" % args.gpus)
