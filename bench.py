@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X published HBM3E bandwidth, bytes/s
+# integer VALU issue peak: 256 CUs, one wave64 instruction per cycle per CU
+# (4 SIMDs of 16 lanes, 4 cycles each), 2.4 GHz peak engine clock
+VALU_PEAK = 256 * 2.4e9    # wave-instructions/s (= 39.3e12 32-bit lane ops/s)
 NPRIMES = 2
 
 
@@ -374,8 +377,9 @@ def main():
     if args.steps is None:
         args.steps = 100 if args.workload == "single" else 20
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (the product path "
-              "shards as described in DESIGN.md, but no multi-GPU run is part of this build)" % args.gpus)
+        print("bench.py: measures one GPU only; --gpus %d is not supported (a product's "
+              "transform columns and rows would be split across GPUs, but no multi-GPU "
+              "path is built)" % args.gpus)
         sys.exit(2)
 
     from preclib_amd import pgpu, workloads
@@ -449,6 +453,22 @@ def main():
         v = pgpu.ntt_plans()[24]
         if st and st["kernel"].replace(" ", "").find("<24,1,%d>" % v) >= 0:
             traffic = st["traffic_bytes"]
+    # VALU roofline: SQ_INSTS_VALU per dispatch (rocprofv3 --pmc, a count
+    # fixed by the plan) over each kernel's measured launch time
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu_2p22.json")
+    if args.logn == 22 and os.path.exists(vpath):
+        vst = json.load(open(vpath))["stages"]
+        v = pgpu.ntt_plans()[24]
+        ok = all(st["kernel"].replace(" ", "").find("<24,1,%d>" % v) >= 0
+                 for k, st in vst.items() if k.startswith("ntt_"))
+        if ok and dom in vst:
+            ins = vst[dom]["valu_wave_instructions"]
+            valu = {"bound": "valu", "kernel": dom, "achieved": ins / avg_s / 1e9, "peak": VALU_PEAK / 1e9,
+                    "unit": "G wave-instructions/s", "frac": ins / avg_s / VALU_PEAK,
+                    "instructions": ins,
+                    "per_kernel_frac": {k: vst[k]["valu_wave_instructions"] / (stages[k][1] / stages[k][0] * 1e-3)
+                                        / VALU_PEAK for k in stages if k in vst}}
     line = {
         "metric": "product limbs/s at 2^22 limbs (plain GPU NTT product, device-resident)",
         "value": value,
@@ -470,6 +490,7 @@ def main():
                      "avg_launch_ms": avg_s * 1e3},
         "product_roofline": {"bytes": prod_bytes, "achieved_GBps": prod_bytes / (ms * 1e-3) / 1e9,
                              "frac": prod_bytes / (ms * 1e-3) / HBM_PEAK},
+        "valu_roofline": valu,
         "stages_ms": {k: v[1] / v[0] for k, v in stages.items()},
     }
     if not args.no_cpu:
