@@ -111,6 +111,10 @@ enum { MODE_REF = 0, MODE_WIDE = 1 };
 // with a tile's operands adjacent, and the same with primes last to first.
 enum { COLF_OP_PRIME = 0, COLF_TILE_PRIME = 1, COLF_TILE_OP = 2, COLF_TILE_OP_REV = 3, ROW_PRIME_REV = 4 };
 constexpr int mode_primes(int mode) { return mode == MODE_WIDE ? 3 : 2; }
+// Index of the prime whose transforms run on lazy residues in [0, 2p): the
+// set's first prime, 469762049 = 7 * 2^26 + 1, the one below 2^30 (the host
+// runtime checks this against its prime table).
+constexpr int LAZY_PRIME = 0;
 constexpr int mode_digit_bits(int mode) { return mode == MODE_WIDE ? 32 : 16; }
 
 struct CrtK {

@@ -39,6 +39,25 @@ __device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint32_t w, uint32_t w
     return min(r, r - p);
 }
 
+// The same product left in [0, 2p) (Shoup's bound before the last
+// correction).
+__device__ __forceinline__ uint32_t mul_shoup_lazy(uint32_t x, uint32_t w, uint32_t wq, uint32_t p) {
+    return x * w - __umulhi(x, wq) * p;
+}
+
+// Lazy residues (LZ): for a prime below 2^30, values are kept in [0, 2p)
+// instead of [0, p) through a whole transform (D. Harvey, "Faster arithmetic
+// for number-theoretic transforms", J. Symbolic Comput. 60 (2014)).  Sums and
+// differences are reduced against 2p, and products by a fixed twiddle skip
+// Shoup's last correction.  Every operand stays below 4p < 2^32.
+constexpr uint32_t LAZY_PRIME_LIMIT = 1u << 30;
+
+template <bool LZ>
+__device__ __forceinline__ uint32_t mul_tw(uint32_t x, uint32_t w, uint32_t wq, uint32_t p) {
+    if constexpr (LZ) return mul_shoup_lazy(x, w, wq, p);
+    else return mul_shoup(x, w, wq, p);
+}
+
 // a * b * 2^-32 mod p for a * b < p * 2^32.
 __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t ninv) {
     uint64_t t = (uint64_t)a * b;
@@ -59,14 +78,14 @@ __device__ __forceinline__ constexpr int bitrev(int q) {
 // R-point forward DFT in registers, Gentleman-Sande DIF: natural order in,
 // x[q] = X[bitrev(q)] out.  Twiddles omega_R^k = omega_32^(k * 32 / R).
 // HALFZ: x[R/2 .. R) are zero on entry, so the first stage is only the
-// twiddle products of the lower half.
-template <int R, bool HALFZ = false>
+// twiddle products of the lower half.  LZ: residues in [0, 2p) in and out.
+template <int R, bool HALFZ = false, bool LZ = false>
 __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
-    const uint32_t p = c.p;
+    const uint32_t p = c.p, pr = LZ ? 2 * c.p : c.p;   // reduction bound
     if constexpr (HALFZ) {
 #pragma unroll
         for (int j = 0; j < R / 2; ++j)
-            x[j + R / 2] = j == 0 ? x[0] : mul_shoup(x[j], c.w32[j * (32 / R)], c.w32q[j * (32 / R)], p);
+            x[j + R / 2] = j == 0 ? x[0] : mul_tw<LZ>(x[j], c.w32[j * (32 / R)], c.w32q[j * (32 / R)], p);
     }
 #pragma unroll
     for (int h = HALFZ ? R / 4 : R / 2; h >= 1; h >>= 1) {
@@ -75,14 +94,14 @@ __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
 #pragma unroll
             for (int j = 0; j < h; ++j) {
                 uint32_t u = x[b + j], v = x[b + j + h];
-                x[b + j] = mod_add(u, v, p);
+                x[b + j] = mod_add(u, v, pr);
                 if (j == 0) {
-                    x[b + j + h] = mod_sub(u, v, p);
+                    x[b + j + h] = mod_sub(u, v, pr);
                 } else {
-                    // u - v + p lies in [1, 2p) < 2^32: Shoup's product takes
-                    // it unreduced
+                    // u - v + pr lies in [1, 2 pr) < 2^32: Shoup's product
+                    // takes it unreduced
                     const int k = j * (16 / h);   // omega_{2h}^j = omega_32^(j*16/h)
-                    x[b + j + h] = mul_shoup(u - v + p, c.w32[k], c.w32q[k], p);
+                    x[b + j + h] = mul_tw<LZ>(u - v + pr, c.w32[k], c.w32q[k], p);
                 }
             }
         }
@@ -90,10 +109,10 @@ __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
 }
 
 // R-point inverse DFT (unnormalized), Cooley-Tukey DIT: x[q] = Y[bitrev(q)]
-// in, natural order out.
-template <int R>
+// in, natural order out.  LZ: residues in [0, 2p) in and out.
+template <int R, bool LZ = false>
 __device__ __forceinline__ void dft_dit_inv(uint32_t (&x)[R], const PrimeK &c) {
-    const uint32_t p = c.p;
+    const uint32_t p = c.p, pr = LZ ? 2 * c.p : c.p;
 #pragma unroll
     for (int h = 1; h <= R / 2; h <<= 1) {
 #pragma unroll
@@ -103,10 +122,10 @@ __device__ __forceinline__ void dft_dit_inv(uint32_t (&x)[R], const PrimeK &c) {
                 uint32_t u = x[b + j], v = x[b + j + h];
                 if (j != 0) {
                     const int k = j * (16 / h);
-                    v = mul_shoup(v, c.iw32[k], c.iw32q[k], p);
+                    v = mul_tw<LZ>(v, c.iw32[k], c.iw32q[k], p);
                 }
-                x[b + j] = mod_add(u, v, p);
-                x[b + j + h] = mod_sub(u, v, p);
+                x[b + j] = mod_add(u, v, pr);
+                x[b + j + h] = mod_sub(u, v, pr);
             }
         }
     }

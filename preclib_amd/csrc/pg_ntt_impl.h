@@ -66,8 +66,9 @@ __device__ __forceinline__ int fresh_tid() {
 // ---- in-LDS radix passes --------------------------------------------------
 
 // HALFZ (first pass only): inputs q >= R/2, the upper half of the
-// transform, are zero and are neither read nor added.
-template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool HALFZ = false>
+// transform, are zero and are neither read nor added.  LZ: lazy residues in
+// [0, 2p) (pg_modarith.h), for a prime below 2^30.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool HALFZ = false, bool LZ = false>
 __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict__ tw, const PrimeK &c, int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
     constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
@@ -80,13 +81,13 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
         uint32_t x[R];
 #pragma unroll
         for (int q = 0; q < (HALFZ ? R / 2 : R); ++q) x[q] = lds[lp(base + q * S)];
-        dft_dif<R, HALFZ>(x, c);
+        dft_dif<R, HALFZ, LZ>(x, c);
         if constexpr (M > 1) {
             const uint2 *t = tw + j2;
 #pragma unroll
             for (int q = 1; q < R; ++q) {
                 const uint2 w = t[bitrev<R>(q) * M];
-                x[q] = mul_shoup(x[q], w.x, w.y, c.p);
+                x[q] = mul_tw<LZ>(x[q], w.x, w.y, c.p);
             }
         }
 #pragma unroll
@@ -96,7 +97,7 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
 
 // pass_fwd over two tiles of the same shape (the row kernel's two operands):
 // each column's twiddles are loaded once and used for both.
-template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool LZ = false>
 __device__ __forceinline__ void pass_fwd2(uint32_t *lda, uint32_t *ldb, const uint2 *__restrict__ tw, const PrimeK &c,
                                           int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
@@ -119,10 +120,10 @@ __device__ __forceinline__ void pass_fwd2(uint32_t *lda, uint32_t *ldb, const ui
             uint32_t x[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
-            dft_dif<R>(x, c);
+            dft_dif<R, false, LZ>(x, c);
             if constexpr (M > 1) {
 #pragma unroll
-                for (int q = 1; q < R; ++q) x[q] = mul_shoup(x[q], w[q].x, w[q].y, c.p);
+                for (int q = 1; q < R; ++q) x[q] = mul_tw<LZ>(x[q], w[q].x, w[q].y, c.p);
             }
 #pragma unroll
             for (int q = 0; q < R; ++q) lds[lp(base + bitrev<R>(q) * S)] = x[q];
@@ -130,7 +131,7 @@ __device__ __forceinline__ void pass_fwd2(uint32_t *lda, uint32_t *ldb, const ui
     }
 }
 
-template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool LZ = false>
 __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict__ itw, const PrimeK &c, int tid) {
     constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
     constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
@@ -148,10 +149,10 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
 #pragma unroll
             for (int q = 1; q < R; ++q) {
                 const uint2 w = t[bitrev<R>(q) * M];
-                x[q] = mul_shoup(x[q], w.x, w.y, c.p);
+                x[q] = mul_tw<LZ>(x[q], w.x, w.y, c.p);
             }
         }
-        dft_dit_inv<R>(x, c);
+        dft_dit_inv<R, LZ>(x, c);
 #pragma unroll
         for (int q = 0; q < R; ++q) lds[lp(base + q * S)] = x[q];
     }
@@ -160,49 +161,49 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
 // Pass lists: forward runs passes 0..np-1 with shrinking block size; the
 // inverse runs them backwards (post-order recursion).  Passes I >= STOP are
 // left out (the row kernel runs its last pass in registers).
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP>
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
 __device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0 && I < STOP) {
-        pass_fwd<LR, LOGN, LOGT, LOGB, NT>(lds, tw[I], c, tid);
+        pass_fwd<LR, LOGN, LOGT, LOGB, NT, false, LZ>(lds, tw[I], c, tid);
         lds_sync();
-        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lds, tw, c, tid);
+        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lds, tw, c, tid);
     }
 }
 
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP>
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
 __device__ __forceinline__ void fwd2_from(uint32_t *lda, uint32_t *ldb, const uint2 *const *tw, const PrimeK &c,
                                           int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0 && I < STOP) {
-        pass_fwd2<LR, LOGN, LOGT, LOGB, NT>(lda, ldb, tw[I], c, tid);
+        pass_fwd2<LR, LOGN, LOGT, LOGB, NT, LZ>(lda, ldb, tw[I], c, tid);
         lds_sync();
-        fwd2_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lda, ldb, tw, c, tid);
+        fwd2_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lda, ldb, tw, c, tid);
     }
 }
 
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP>
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
 __device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
     constexpr int LR = split_lr(LOGN, I, RB);
     if constexpr (LR > 0 && I < STOP) {
-        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP>(lds, itw, c, tid);
-        pass_inv<LR, LOGN, LOGT, LOGB, NT>(lds, itw[I], c, tid);
+        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lds, itw, c, tid);
+        pass_inv<LR, LOGN, LOGT, LOGB, NT, LZ>(lds, itw[I], c, tid);
         lds_sync();
     }
 }
 
 // tid: the thread index (callers may pass an opaque copy of threadIdx.x so
 // that index arithmetic is recomputed per call rather than kept live).
-template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS>
+template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS, bool LZ = false>
 __device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c,
                                             int tid = threadIdx.x) {
-    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP>(lds, tw, c, tid);
+    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lds, tw, c, tid);
 }
 
-template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS>
+template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS, bool LZ = false>
 __device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c,
                                             int tid = threadIdx.x) {
-    inv_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP>(lds, itw, c, tid);
+    inv_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lds, itw, c, tid);
 }
 
 // Frequency index held at position rho after the forward passes of a 2^LOGN
@@ -291,6 +292,20 @@ __device__ __forceinline__ void put_digits(uint32_t *lds, int i, uint64_t v, uin
 // (COLF_*): with COLF_TILE_PRIME the workgroups of one tile's primes are dealt
 // to the same XCD one after another (xcd_tile_prime), so the operand limbs
 // that the first reads from HBM the others find in L2.
+// Column transforms of a staged tile (half: the upper half of every column
+// is zero).
+template <int L1, int LT, int RBV, bool LZ>
+__device__ __forceinline__ void col_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, bool half) {
+    constexpr int LR0 = split_lr(L1, 0, RBV);
+    if (half) {
+        pass_fwd<LR0, L1, LT, L1, NTT_THREADS, true, LZ>(lds, tw[0], c, threadIdx.x);
+        lds_sync();
+        fwd_from<L1, LT, NTT_THREADS, L1 - LR0, 1, RBV, MAX_PASS, LZ>(lds, tw, c, threadIdx.x);
+    } else {
+        lds_forward<L1, LT, NTT_THREADS, RBV, MAX_PASS, LZ>(lds, tw, c);
+    }
+}
+
 template <int LOGN, int MODE, int V>
 __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(8))) k_col_fwd(const NttLaunch L) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
@@ -338,13 +353,8 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
         }
         lds_sync();
         if constexpr (LR0 > 0) {
-            if (half) {
-                pass_fwd<LR0, L1, LT, L1, NTT_THREADS, true>(lds, L.col_tw[pr][0], c, threadIdx.x);
-                lds_sync();
-                fwd_from<L1, LT, NTT_THREADS, L1 - LR0, 1, RBV, MAX_PASS>(lds, L.col_tw[pr], c, threadIdx.x);
-            } else {
-                lds_forward<L1, LT, NTT_THREADS, RBV>(lds, L.col_tw[pr], c);
-            }
+            if (pr == LAZY_PRIME) col_forward<L1, LT, RBV, true>(lds, L.col_tw[pr], c, half);
+            else col_forward<L1, LT, RBV, false>(lds, L.col_tw[pr], c, half);
         }
         uint32_t *out = D.spec + ((size_t)(op * NP + pr) << LOGN);
         if constexpr (TW::STEP) {
@@ -392,9 +402,8 @@ __device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, cons
     }
 }
 
-template <int LOGN, int MODE, int V>
-__global__ void __launch_bounds__(row_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(row_waves(LOGN, V))))
-k_row(const NttLaunch L) {
+template <int LOGN, int MODE, int V, bool LZ>
+__device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, int pr, int row) {
     constexpr int L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
     constexpr int RBV = plan_rb(LOGN, V);
     // The last forward pass (block size R_L, no twiddles) runs in registers:
@@ -405,8 +414,6 @@ k_row(const NttLaunch L) {
     constexpr int LRL = split_lr(L2, NPS - 1, RBV), RL = 1 << LRL;
     constexpr int NCL = N2 >> LRL, IL = (NCL + ROW_THREADS - 1) / ROW_THREADS;
     extern __shared__ uint32_t lds[];
-    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
-    const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
     const PrimeK &c = L.pk[pr];
     if constexpr (row_dual(LOGN, V)) {
         // Both operands' rows in LDS (A at 0, B at N2): every forward pass
@@ -417,7 +424,7 @@ k_row(const NttLaunch L) {
         load_row<LOGN, MODE, V>(lda, L, D, 0, pr, row);
         load_row<LOGN, MODE, V>(ldb, L, D, 1, pr, row);
         lds_sync();
-        fwd2_from<L2, 0, ROW_THREADS, L2, 0, RBV, NPS - 1>(lda, ldb, L.row_tw[pr], c, threadIdx.x);
+        fwd2_from<L2, 0, ROW_THREADS, L2, 0, RBV, NPS - 1, LZ>(lda, ldb, L.row_tw[pr], c, threadIdx.x);
 #pragma unroll
         for (int it = 0; it < IL; ++it) {
             const int col = threadIdx.x + it * ROW_THREADS;
@@ -425,19 +432,20 @@ k_row(const NttLaunch L) {
                 uint32_t xa[RL], x[RL];
 #pragma unroll
                 for (int q = 0; q < RL; ++q) xa[q] = lda[lp(col * RL + q)];
-                dft_dif<RL>(xa, c);
+                dft_dif<RL, false, LZ>(xa, c);
 #pragma unroll
                 for (int q = 0; q < RL; ++q) x[q] = ldb[lp(col * RL + q)];
-                dft_dif<RL>(x, c);
+                dft_dif<RL, false, LZ>(x, c);
+                // lazy operands: x xa < 4p^2 < p 2^32
 #pragma unroll
                 for (int q = 0; q < RL; ++q) x[q] = mont_mul(x[q], xa[q], c.p, c.ninv);
-                dft_dit_inv<RL>(x, c);
+                dft_dit_inv<RL, LZ>(x, c);
 #pragma unroll
                 for (int q = 0; q < RL; ++q) ldb[lp(col * RL + q)] = x[q];
             }
         }
         lds_sync();
-        lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1>(ldb, L.row_itw[pr], c);
+        lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(ldb, L.row_itw[pr], c);
         uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
             out[m] = make_uint4(ldb[lp(4 * m)], ldb[lp(4 * m + 1)], ldb[lp(4 * m + 2)], ldb[lp(4 * m + 3)]);
@@ -460,14 +468,14 @@ k_row(const NttLaunch L) {
         }
     }
     lds_sync();
-    lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1>(lds, L.row_tw[pr], c);
+    lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_tw[pr], c);
 #pragma unroll
     for (int it = 0; it < IL; ++it) {
         const int col = threadIdx.x + it * ROW_THREADS;
         if (col < NCL) {
 #pragma unroll
             for (int q = 0; q < RL; ++q) keep[it][q] = lds[lp(col * RL + q)];
-            dft_dif<RL>(keep[it], c);
+            dft_dif<RL, false, LZ>(keep[it], c);
         }
     }
     lds_sync();
@@ -484,7 +492,7 @@ k_row(const NttLaunch L) {
         load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
     }
     lds_sync();
-    lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1>(lds, L.row_tw[pr], c);
+    lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_tw[pr], c);
     // B's last pass, pointwise product and the inverse's first pass share the
     // registers; every thread reads its inputs before any writes its outputs
     // to the same positions, within one column
@@ -495,19 +503,31 @@ k_row(const NttLaunch L) {
             uint32_t x[RL];
 #pragma unroll
             for (int q = 0; q < RL; ++q) x[q] = lds[lp(col * RL + q)];
-            dft_dif<RL>(x, c);
+            dft_dif<RL, false, LZ>(x, c);
 #pragma unroll
             for (int q = 0; q < RL; ++q) x[q] = mont_mul(x[q], keep[it][q], c.p, c.ninv);
-            dft_dit_inv<RL>(x, c);
+            dft_dit_inv<RL, LZ>(x, c);
 #pragma unroll
             for (int q = 0; q < RL; ++q) lds[lp(col * RL + q)] = x[q];
         }
     }
     lds_sync();
-    lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1>(lds, L.row_itw[pr], c);
+    lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_itw[pr], c);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
         out[m] = make_uint4(lds[lp(4 * m)], lds[lp(4 * m + 1)], lds[lp(4 * m + 2)], lds[lp(4 * m + 3)]);
+}
+
+// Row DFTs of both operands, pointwise product, inverse row DFT.  The lazy
+// prime's residues leave in [0, 2p); the inverse column kernel's first
+// Montgomery product takes them as they are.
+template <int LOGN, int MODE, int V>
+__global__ void __launch_bounds__(row_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(row_waves(LOGN, V))))
+k_row(const NttLaunch L) {
+    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
+    if (pr == LAZY_PRIME) row_body<LOGN, MODE, V, true>(L, D, pr, row);
+    else row_body<LOGN, MODE, V, false>(L, D, pr, row);
 }
 
 // ---- kernel 3: inverse column pass, CRT, limb sums ------------------------------
@@ -570,13 +590,17 @@ __device__ __forceinline__ void stage_prime(uint32_t *lds, const NttLaunch &L, c
 
 // Prime pr's residues through the inverse column transform; the result is in
 // LDS (after a workgroup barrier).
+// The lazy prime's residues come out in [0, 2p).
 template <int LOGN, int V>
 __device__ __forceinline__ void inv_column(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int pr, uint32_t c0,
                                            int tid = threadIdx.x) {
-    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
     stage_prime<LOGN, V>(lds, L, D, pr, c0, tid);
     lds_sync();
-    if constexpr (L1 > 0) lds_inverse<L1, LT, COLINV_THREADS, plan_rb(LOGN, V)>(lds, L.col_itw[pr], L.pk[pr], tid);
+    if constexpr (L1 > 0) {
+        if (pr == LAZY_PRIME) lds_inverse<L1, LT, COLINV_THREADS, RBV, MAX_PASS, true>(lds, L.col_itw[pr], L.pk[pr], tid);
+        else lds_inverse<L1, LT, COLINV_THREADS, RBV, MAX_PASS, false>(lds, L.col_itw[pr], L.pk[pr], tid);
+    }
 }
 
 // Value of lane (this lane ^ 1), by a DPP quad permutation [1, 0, 3, 2].
@@ -610,10 +634,12 @@ __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, 
         uint32_t *lds_p = lds;
         inv_column<LOGN, V>(lds_p, L, D, pr, c0, tid);
         if (pr == 0) {
+            const uint32_t p1 = L.pk[0].p;
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int e = tid + k * COLINV_THREADS;
-                r0[k] = e < NE ? lds_p[lp(e)] : 0;
+                const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
+                r0[k] = min(r, r - p1);   // lazy prime: [0, 2p) -> [0, p)
             }
         } else if (pr == 1) {
             const uint32_t p2 = L.pk[1].p;
@@ -677,7 +703,7 @@ __device__ __forceinline__ void col_inv_ref(uint32_t *lds, const NttLaunch &L, c
             if (i < NE) {
                 const uint32_t r = lds[lp(i)];
                 if (pr == 0) {
-                    res0[k] = r;
+                    res0[k] = min(r, r - L.pk[0].p);   // lazy prime: [0, 2p) -> [0, p)
                 } else {
                     // Garner (mul_ntt.cpp:405-412): x = r1 + p1 * ((r2 - r1) * p1^-1 mod p2)
                     const uint32_t d = mod_sub(r, res0[k], L.pk[1].p);   // r1 < p1 < p2

@@ -12,6 +12,7 @@ namespace pg {
 // NTT primes and generators (src/mul_ntt.cpp:29-34).
 constexpr uint32_t PRIME[3] = {469762049u, 1811939329u, 2013265921u};
 constexpr uint32_t GEN[3] = {3u, 13u, 31u};
+static_assert(PRIME[LAZY_PRIME] < LAZY_PRIME_LIMIT, "lazy residues need 4p < 2^32");
 
 uint32_t powmod(uint32_t b, uint64_t e, uint32_t p);
 uint32_t shoup_q(uint32_t w, uint32_t p);
