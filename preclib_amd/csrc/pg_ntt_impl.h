@@ -288,10 +288,6 @@ __device__ __forceinline__ void put_digits(uint32_t *lds, int i, uint64_t v, uin
 
 // ---- kernel 1: forward column pass ------------------------------------------
 
-// One workgroup per (tile, prime, operand), in the order L.order selects
-// (COLF_*): with COLF_TILE_PRIME the workgroups of one tile's primes are dealt
-// to the same XCD one after another (xcd_tile_prime), so the operand limbs
-// that the first reads from HBM the others find in L2.
 // Column transforms of a staged tile (half: the upper half of every column
 // is zero).
 template <int L1, int LT, int RBV, bool LZ>
@@ -306,6 +302,10 @@ __device__ __forceinline__ void col_forward(uint32_t *lds, const uint2 *const *t
     }
 }
 
+// One workgroup per (tile, prime, operand), in the order L.order selects
+// (COLF_*): with COLF_TILE_PRIME the workgroups of one tile's primes are dealt
+// to the same XCD one after another (xcd_tile_prime), so the operand limbs
+// that the first reads from HBM the others find in L2.
 template <int LOGN, int MODE, int V>
 __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(8))) k_col_fwd(const NttLaunch L) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
