@@ -4,8 +4,8 @@
 // so a sum of two residues fits in 32 bits and every reduction below is a
 // single unsigned min().  Two multiplication forms are used:
 //  * Shoup's precomputed-quotient product x*w mod p for fixed twiddles
-//    (wq = floor(w * 2^32 / p)): 1 mul_hi + 2 mul_lo, result in [0, 2p),
-//    then one min() to [0, p).
+//    (wq = floor(w * 2^32 / p)): 1 mul_hi, 1 mul_lo and 1 mad_u64_u32,
+//    result in [0, 2p), then one min() to [0, p).
 //  * Montgomery's product a*b*2^-32 mod p (the reference's mont_reduce,
 //    mul_ntt.cpp:43-48) for the pointwise spectrum product and for twiddles
 //    formed on the fly, which have no precomputed quotient.
@@ -18,6 +18,7 @@ namespace pg {
 struct PrimeK {
     uint32_t p;      // modulus
     uint32_t ninv;   // -p^-1 mod 2^32
+    uint32_t np;     // 2^32 - p
     uint32_t w32[16], w32q[16];     // omega_32^k, Shoup quotients (forward)
     uint32_t iw32[16], iw32q[16];   // omega_32^-k, Shoup quotients (inverse)
 };
@@ -32,19 +33,6 @@ __device__ __forceinline__ uint32_t mod_sub(uint32_t a, uint32_t b, uint32_t p) 
     return min(d, d + p);
 }
 
-// x * w mod p with wq = floor(w * 2^32 / p); x < 2^32, w < p.
-__device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint32_t w, uint32_t wq, uint32_t p) {
-    uint32_t q = __umulhi(x, wq);
-    uint32_t r = x * w - q * p;
-    return min(r, r - p);
-}
-
-// The same product left in [0, 2p) (Shoup's bound before the last
-// correction).
-__device__ __forceinline__ uint32_t mul_shoup_lazy(uint32_t x, uint32_t w, uint32_t wq, uint32_t p) {
-    return x * w - __umulhi(x, wq) * p;
-}
-
 // Lazy residues (LZ): for a prime below 2^30, values are kept in [0, 2p)
 // instead of [0, p) through a whole transform (D. Harvey, "Faster arithmetic
 // for number-theoretic transforms", J. Symbolic Comput. 60 (2014)).  Sums and
@@ -52,10 +40,17 @@ __device__ __forceinline__ uint32_t mul_shoup_lazy(uint32_t x, uint32_t w, uint3
 // Shoup's last correction.  Every operand stays below 4p < 2^32.
 constexpr uint32_t LAZY_PRIME_LIMIT = 1u << 30;
 
-template <bool LZ>
-__device__ __forceinline__ uint32_t mul_tw(uint32_t x, uint32_t w, uint32_t wq, uint32_t p) {
-    if constexpr (LZ) return mul_shoup_lazy(x, w, wq, p);
-    else return mul_shoup(x, w, wq, p);
+// x * w mod c.p with wq = floor(w * 2^32 / p); x < 2^32, w < p.  With
+// q = floor(x wq / 2^32), x w - q p lies in [0, 2p); one min() brings it to
+// [0, p) (LZ: left in [0, 2p)).  x w - q p is formed as x w + q (2^32 - p)
+// with the negated modulus read from the prime's constants: the compiler
+// then emits one v_mad_u64_u32 (full rate on gfx950) for the multiply-add
+// instead of a mul_lo and a subtract.
+template <bool LZ = false>
+__device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint32_t w, uint32_t wq, const PrimeK &c) {
+    const uint32_t r = x * w + __umulhi(x, wq) * c.np;   // [0, 2p)
+    if constexpr (LZ) return r;
+    else return min(r, r - c.p);
 }
 
 // a * b * 2^-32 mod p for a * b < p * 2^32.
@@ -85,7 +80,7 @@ __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
     if constexpr (HALFZ) {
 #pragma unroll
         for (int j = 0; j < R / 2; ++j)
-            x[j + R / 2] = j == 0 ? x[0] : mul_tw<LZ>(x[j], c.w32[j * (32 / R)], c.w32q[j * (32 / R)], p);
+            x[j + R / 2] = j == 0 ? x[0] : mul_shoup<LZ>(x[j], c.w32[j * (32 / R)], c.w32q[j * (32 / R)], c);
     }
 #pragma unroll
     for (int h = HALFZ ? R / 4 : R / 2; h >= 1; h >>= 1) {
@@ -101,7 +96,7 @@ __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
                     // u - v + pr lies in [1, 2 pr) < 2^32: Shoup's product
                     // takes it unreduced
                     const int k = j * (16 / h);   // omega_{2h}^j = omega_32^(j*16/h)
-                    x[b + j + h] = mul_tw<LZ>(u - v + pr, c.w32[k], c.w32q[k], p);
+                    x[b + j + h] = mul_shoup<LZ>(u - v + pr, c.w32[k], c.w32q[k], c);
                 }
             }
         }
@@ -122,7 +117,7 @@ __device__ __forceinline__ void dft_dit_inv(uint32_t (&x)[R], const PrimeK &c) {
                 uint32_t u = x[b + j], v = x[b + j + h];
                 if (j != 0) {
                     const int k = j * (16 / h);
-                    v = mul_tw<LZ>(v, c.iw32[k], c.iw32q[k], p);
+                    v = mul_shoup<LZ>(v, c.iw32[k], c.iw32q[k], c);
                 }
                 x[b + j] = mod_add(u, v, pr);
                 x[b + j + h] = mod_sub(u, v, pr);

@@ -87,7 +87,7 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
 #pragma unroll
             for (int q = 1; q < R; ++q) {
                 const uint2 w = t[bitrev<R>(q) * M];
-                x[q] = mul_tw<LZ>(x[q], w.x, w.y, c.p);
+                x[q] = mul_shoup<LZ>(x[q], w.x, w.y, c);
             }
         }
 #pragma unroll
@@ -123,7 +123,7 @@ __device__ __forceinline__ void pass_fwd2(uint32_t *lda, uint32_t *ldb, const ui
             dft_dif<R, false, LZ>(x, c);
             if constexpr (M > 1) {
 #pragma unroll
-                for (int q = 1; q < R; ++q) x[q] = mul_tw<LZ>(x[q], w[q].x, w[q].y, c.p);
+                for (int q = 1; q < R; ++q) x[q] = mul_shoup<LZ>(x[q], w[q].x, w[q].y, c);
             }
 #pragma unroll
             for (int q = 0; q < R; ++q) lds[lp(base + bitrev<R>(q) * S)] = x[q];
@@ -149,7 +149,7 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
 #pragma unroll
             for (int q = 1; q < R; ++q) {
                 const uint2 w = t[bitrev<R>(q) * M];
-                x[q] = mul_tw<LZ>(x[q], w.x, w.y, c.p);
+                x[q] = mul_shoup<LZ>(x[q], w.x, w.y, c);
             }
         }
         dft_dit_inv<R, LZ>(x, c);
@@ -647,7 +647,7 @@ __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, 
             for (int k = 0; k < E; ++k) {
                 const int e = tid + k * COLINV_THREADS;
                 const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
-                t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, p2);
+                t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, L.pk[1]);
             }
         } else {
             const uint32_t p3 = L.pk[2].p;
@@ -658,8 +658,8 @@ __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, 
                 const int e = tid + k * COLINV_THREADS;
                 const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
                 // (r1 + p1 t2) mod p3, with r1 < p1 < p3 and t2 < p2 < p3
-                const uint32_t x12m = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, p3), p3);
-                const uint32_t t3 = mul_shoup(mod_sub(r, x12m, p3), K.inv123, K.inv123q, p3);
+                const uint32_t x12m = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, L.pk[2]), p3);
+                const uint32_t t3 = mul_shoup(mod_sub(r, x12m, p3), K.inv123, K.inv123q, L.pk[2]);
                 // x = a + 2^32 b
                 const uint64_t x12 = (uint64_t)r0[k] + p1 * t2[k];
                 const uint64_t s0 = x12 + (uint64_t)p12lo * t3;
@@ -707,7 +707,7 @@ __device__ __forceinline__ void col_inv_ref(uint32_t *lds, const NttLaunch &L, c
                 } else {
                     // Garner (mul_ntt.cpp:405-412): x = r1 + p1 * ((r2 - r1) * p1^-1 mod p2)
                     const uint32_t d = mod_sub(r, res0[k], L.pk[1].p);   // r1 < p1 < p2
-                    const uint32_t t = mul_shoup(d, K.inv12, K.inv12q, L.pk[1].p);
+                    const uint32_t t = mul_shoup(d, K.inv12, K.inv12q, L.pk[1]);
                     coef[k] = (uint64_t)res0[k] + (uint64_t)L.pk[0].p * t;
                 }
             }

@@ -37,6 +37,7 @@ PrimeK make_prime(int idx) {
     for (int i = 0; i < 5; ++i) x *= 2u - p * x;   // p^-1 mod 2^32
     k.p = p;
     k.ninv = 0u - x;
+    k.np = 0u - p;
     const uint32_t w = powmod(GEN[idx], (p - 1) / 32, p);
     const uint32_t iw = powmod(w, p - 2, p);
     uint32_t a = 1, b = 1;
