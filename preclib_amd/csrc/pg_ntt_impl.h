@@ -613,8 +613,11 @@ __device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
 //   x  = r1 + p1 t2 + p1 p2 t3 < 2^89 (the exact coefficient);
 // a limb is columns (2m, 2m+1) of a row, x_even + 2^32 x_odd, held by lanes
 // (2i, 2i+1), so the odd lane's coefficient reaches its partner by DPP and the
-// even lane writes the 128-bit limb sum.  At most 64 VGPRs, so two
-// workgroups share a CU.
+// even lane writes the 128-bit limb sum.  With at most 8 elements per
+// thread this fits 64 VGPRs and two workgroups share a CU (col_inv_waves).
+// (Halved tiles measured slower at 2^24 points: with 1024 threads they spill
+// at 64 VGPRs; with 512 the occupancy stays and the rows are read in 16-byte
+// pieces.)
 template <int LOGN, int V>
 __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, uint32_t c0) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
