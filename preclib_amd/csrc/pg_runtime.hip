@@ -3,6 +3,7 @@
 #include "pg_runtime.h"
 #include <dlfcn.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -653,14 +654,27 @@ int autotune_ntt(int lo, int hi, int reps, double *ms, int *chosen) {
             }
             if (!st) st = C.sync();
             if (st) break;
+            // The GPU clocks ramp over ~30 ms of load: warm up for at least
+            // that long, then time at least 10 ms (and reps products).
+            float t = 0.f;
             HIP_TRY(hipEventRecord(e0, C.stream));
-            for (int k = 0; k < reps && !st; ++k) st = ntt_mul_device(out, a, an, ~0ull, a + an, bn, ~0ull, MODE_WIDE);
+            st = ntt_mul_device(out, a, an, ~0ull, a + an, bn, ~0ull, MODE_WIDE);
             HIP_TRY(hipEventRecord(e1, C.stream));
             if (!st) st = C.sync();
             if (st) break;
-            float t = 0.f;
             HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-            ms[row * NTT_VARIANTS + v] = t / reps;
+            const double one = std::max(0.001, (double)t);
+            const int warm = (int)std::min(2000.0, std::ceil(30.0 / one));
+            const int timed = std::max(reps, (int)std::min(2000.0, std::ceil(10.0 / one)));
+            for (int k = 0; k < warm && !st; ++k) st = ntt_mul_device(out, a, an, ~0ull, a + an, bn, ~0ull, MODE_WIDE);
+            HIP_TRY(hipEventRecord(e0, C.stream));
+            for (int k = 0; k < timed && !st; ++k) st = ntt_mul_device(out, a, an, ~0ull, a + an, bn, ~0ull, MODE_WIDE);
+            HIP_TRY(hipEventRecord(e1, C.stream));
+            if (!st) st = C.sync();
+            if (st) break;
+            HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+            t /= (float)timed;
+            ms[row * NTT_VARIANTS + v] = t;
             if (v) {   // every variant must give the default plan's product
                 got.resize(an + bn);
                 HIP_TRY(hipMemcpyAsync(got.data(), r, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream));
@@ -671,8 +685,8 @@ int autotune_ntt(int lo, int hi, int reps, double *ms, int *chosen) {
                 HIP_TRY(hipMemcpyAsync(ref.data(), r0, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream));
                 if ((st = C.sync())) break;
             }
-            if (v == 0 || t / reps < best) {
-                best = t / reps;
+            if (v == 0 || t < best) {
+                best = t;
                 chosen[row] = v;
             }
         }
