@@ -60,7 +60,7 @@ private:
         const CarryNode *dnodes = nullptr;
         const uint32_t *dchunk = nullptr;
         uint8_t *agg = nullptr;
-        std::vector<CarryNode> host;
+        double bytes = 0.0;   // carry traffic, for the stage timing
     };
     SumRef new_sums(size_t len);
     int plan(const uint64_t *x, size_t xn, const uint64_t *y, size_t yn, int path, int job, int depth,

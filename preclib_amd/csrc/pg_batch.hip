@@ -24,7 +24,7 @@
 
 namespace pg {
 
-int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dnodes, const uint32_t *dchunk,
+int launch_carry_nodes(size_t nnodes, double bytes, const CarryNode *dnodes, const uint32_t *dchunk,
                        size_t nchunks, uint8_t *agg);
 
 namespace {
@@ -240,9 +240,11 @@ int Batch::build(const pg_job *jobs, size_t n) {
     auto SLO = [&](const SumRef &s) { return (uint64_t *)P(s.lo); };
     auto SHI = [&](const SumRef &s) { return (uint64_t *)P(s.hi); };
 
-    // host images of the descriptor arrays
-    std::vector<char> img(arena_bytes - d_ntt);
+    // host images of the descriptor arrays (every descriptor is written
+    // below; the chunk aggregates start at zero)
+    HostImg img = C.take_img(arena_bytes - d_ntt);
     auto H = [&](size_t off) { return img.data() + (off - d_ntt); };
+    std::memset(H(d_agg), 0, nchunks);
 
     // NTT buckets
     size_t k = 0;
@@ -292,7 +294,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
     }
     // small leaves
     size_t blk = 0;
-    std::vector<uint32_t> smap;
+    uint32_t *smap = (uint32_t *)H(d_smallmap);
     for (size_t i = 0; i < small.size(); ++i) {
         const SmallPlan &sp = small[i];
         SmallLeaf L{};
@@ -302,11 +304,10 @@ int Batch::build(const pg_job *jobs, size_t n) {
         L.blk0 = blk;
         std::memcpy(H(d_small) + i * sizeof(SmallLeaf), &L, sizeof L);
         const size_t nb = small_blocks(sp.xn, sp.yn);
-        for (size_t q = 0; q < nb; ++q) smap.push_back((uint32_t)i);
+        for (size_t q = 0; q < nb; ++q) smap[blk + q] = (uint32_t)i;
         blk += nb;
         small_io += (double)(sp.xn + sp.yn) * 16.0;
     }
-    std::memcpy(H(d_smallmap), smap.data(), smap.size() * 4);
     small_leaves = (SmallLeaf *)P(d_small);
     small_map = (uint32_t *)P(d_smallmap);
     small_nblocks = blk;
@@ -316,7 +317,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
         Level lv;
         lv.first_chunk = chunk;
         lv.first_node = nidx;
-        std::vector<CarryNode> lvnodes;
+        size_t count = 0;   // nodes of this level, written straight into the image
         for (auto &nd : nodes) {
             if (nd.depth != d) continue;
             CarryNode N{};
@@ -347,21 +348,19 @@ int Batch::build(const pg_job *jobs, size_t n) {
             }
             N.chunk0 = chunk - lv.first_chunk;   // relative to the level's launch
             const size_t nc = (std::max(N.m, N.out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
-            for (size_t q = 0; q < nc; ++q) {
-                const uint32_t local = (uint32_t)lvnodes.size();
-                std::memcpy(H(d_chunkmap) + (chunk + q) * 4, &local, 4);
-            }
+            uint32_t *cmap = (uint32_t *)H(d_chunkmap) + chunk;
+            for (size_t q = 0; q < nc; ++q) cmap[q] = (uint32_t)count;
             chunk += nc;
-            lvnodes.push_back(N);
+            std::memcpy(H(d_nodes) + (nidx + count) * sizeof(CarryNode), &N, sizeof N);
+            lv.bytes += (double)N.m * 16.0 * (N.kind == NODE_BLOCKED ? 2 : 1) + (double)N.out_len * 8.0;
+            ++count;
         }
-        std::memcpy(H(d_nodes) + nidx * sizeof(CarryNode), lvnodes.data(), lvnodes.size() * sizeof(CarryNode));
-        lv.count = lvnodes.size();
+        lv.count = count;
         lv.nchunks = chunk - lv.first_chunk;
         lv.dnodes = (const CarryNode *)(P(d_nodes) + nidx * sizeof(CarryNode));
         lv.dchunk = (const uint32_t *)(P(d_chunkmap) + lv.first_chunk * 4);
         lv.agg = (uint8_t *)P(d_agg) + lv.first_chunk;
-        lv.host = lvnodes;
-        nidx += lvnodes.size();
+        nidx += count;
         if (lv.count) levels.push_back(lv);
     }
     // launch descriptors and twiddle tables now: table creation allocates and
@@ -416,7 +415,7 @@ int Batch::enqueue() {
             return set_error(PG_EDEVICE, "pg_batch: small-product launch failed");
     }
     for (auto &lv : levels) {
-        int st = launch_carry_nodes(lv.host, lv.dnodes, lv.dchunk, lv.nchunks, lv.agg);
+        int st = launch_carry_nodes(lv.count, lv.bytes, lv.dnodes, lv.dchunk, lv.nchunks, lv.agg);
         if (st) return st;
     }
     return PG_OK;

@@ -84,7 +84,7 @@ int lin_run(const CarryNode *nodes, size_t n, const char *stage) {
                             (double)v[i].out_len);
         }
         const size_t nb = n * sizeof(CarryNode), mb = map.size() * sizeof(uint32_t);
-        std::vector<char> img(nb + mb);
+        HostImg img = C.take_img(nb + mb);
         std::memcpy(img.data(), v.data(), nb);
         std::memcpy(img.data() + nb, map.data(), mb);
         if (!desc.reset((img.size() + 7) / 8)) return set_error(PG_ENOMEM, "linear descriptors");
@@ -114,7 +114,7 @@ int dset(uint64_t *r, size_t n, const uint64_t *value, size_t vn) {
     Context &C = ctx();
     vn = std::min(vn, n);
     if (vn) {
-        std::vector<char> img(vn * 8);
+        HostImg img = C.take_img(vn * 8);
         std::memcpy(img.data(), value, vn * 8);
         HIP_TRY(hipMemcpyAsync(r, img.data(), vn * 8, hipMemcpyHostToDevice, C.stream));
         C.hold(std::move(img));

@@ -196,7 +196,8 @@ template <class T>
 int upload(DBuf &dst, const std::vector<T> &v) {
     const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 8);
     if (!dst.reset((bytes + 7) / 8)) return set_error(PG_ENOMEM, "descriptor upload");
-    std::vector<char> img(bytes, 0);
+    HostImg img = ctx().take_img(bytes);
+    std::memset(img.data(), 0, bytes);
     if (!v.empty()) std::memcpy(img.data(), v.data(), v.size() * sizeof(T));
     HIP_TRY(hipMemcpyAsync(dst.d, img.data(), bytes, hipMemcpyHostToDevice, ctx().stream));
     ctx().hold(std::move(img));

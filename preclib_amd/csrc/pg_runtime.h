@@ -2,12 +2,36 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <map>
+#include <memory>
+#include <utility>
 #include <mutex>
 #include <string>
 #include <vector>
 #include "pg_kernels.h"
 
 namespace pg {
+
+// Allocator whose value-construction leaves the bytes as they are: resizing a
+// HostImg does not zero it (every byte that matters is written by its user).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+using HostImg = std::vector<char, NoInitAlloc<char>>;
 
 // NTT primes and generators (src/mul_ntt.cpp:29-34).
 constexpr uint32_t PRIME[3] = {469762049u, 1811939329u, 2013265921u};
@@ -100,9 +124,12 @@ public:
     uint64_t *words = nullptr;    // device flags / small results of linear steps
 
     // Host descriptor images handed to hipMemcpyAsync stay alive until the
-    // next sync() of the stream.
-    std::vector<std::vector<char>> held;
-    void hold(std::vector<char> &&img) { held.push_back(std::move(img)); }
+    // next sync() of the stream; their buffers are then kept for reuse, so a
+    // later image of similar size needs neither fresh pages nor a zero fill.
+    std::vector<HostImg> held, spare;
+    HostImg take_img(size_t bytes);
+    void hold(HostImg &&img) { held.push_back(std::move(img)); }
+    void recycle_held();
     int sync();
 
     // timing

@@ -171,6 +171,7 @@ void Context::shutdown() {
     if (words) hipFree(words);
     words = nullptr;
     held.clear();
+    spare.clear();
     pool.release_all();
     hipStreamDestroy(stream);
     stream = nullptr;
@@ -204,8 +205,35 @@ void Context::load_plan_table() {
 
 int Context::sync() {
     if (hipStreamSynchronize(stream) != hipSuccess) return set_error(PG_EDEVICE, "device error");
-    held.clear();
+    recycle_held();
     return PG_OK;
+}
+
+// The stream has passed every upload from the held images: keep their buffers
+// (up to 256 MiB, largest first) for take_img.
+void Context::recycle_held() {
+    for (auto &h : held) spare.push_back(std::move(h));
+    held.clear();
+    std::sort(spare.begin(), spare.end(), [](const HostImg &a, const HostImg &b) { return a.capacity() > b.capacity(); });
+    size_t kept = 0, n = 0;
+    while (n < spare.size() && kept + spare[n].capacity() <= ((size_t)256 << 20)) kept += spare[n++].capacity();
+    spare.resize(n);
+}
+
+HostImg Context::take_img(size_t bytes) {
+    // the smallest spare buffer that holds `bytes` (spare is sorted by
+    // decreasing capacity)
+    for (size_t i = spare.size(); i-- > 0;) {
+        if (spare[i].capacity() >= bytes) {
+            HostImg img = std::move(spare[i]);
+            spare.erase(spare.begin() + (long)i);
+            img.resize(bytes);
+            return img;
+        }
+    }
+    HostImg img;
+    img.resize(bytes);
+    return img;
 }
 
 void *Context::table_alloc(size_t bytes) {
@@ -500,17 +528,15 @@ int fill_fft_tables(FftLaunch &L) {
     return PG_OK;
 }
 
-int launch_carry_nodes(const std::vector<CarryNode> &nodes, const CarryNode *dnodes, const uint32_t *dchunk,
+int launch_carry_nodes(size_t nnodes, double bytes, const CarryNode *dnodes, const uint32_t *dchunk,
                        size_t nchunks, uint8_t *agg) {
     Context &C = ctx();
     CarryLaunch cl{};
     cl.nodes = dnodes;
     cl.chunk_node = dchunk;
     cl.nchunks = nchunks;
-    cl.nnodes = nodes.size();
+    cl.nnodes = nnodes;
     cl.agg = agg;
-    double bytes = 0;
-    for (auto &n : nodes) bytes += (double)n.m * 16.0 * (n.kind == NODE_BLOCKED ? 2 : 1) + (double)n.out_len * 8.0;
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
     HIP_TRY(launch_carry_kernels(cl, C.stream));
