@@ -29,11 +29,14 @@ enum {
     PG_EINTERNAL = -6  /* a checked invariant failed (result not produced)   */
 };
 
-/* Product paths.  PG_PATH_AUTO applies preclib's dispatcher rules for sizes
- * above 192 limbs (mul_basic.cpp:238-276): blocked when hi > 2*lo, NTT when
- * hi > 6144, FFT otherwise.  PG_PATH_NTT / PG_PATH_FFT force the transform
- * (mul_ntt / mul_fft semantics, including mul_ntt's fall back to the FFT
- * above 2^24 limbs or 2^26 points, mul_ntt.cpp:497-506). */
+/* Product paths.  PG_PATH_AUTO applies preclib's dispatcher structure
+ * (mul_basic.cpp:238-276): schoolbook when a side is <= 24 limbs or hi <= 192,
+ * blocked when hi > 2*lo, else a transform.  Where preclib's CPU thresholds
+ * pick mul_fft (hi <= 6144, and every square), the GPU takes the NTT, which
+ * is 2-2.5x faster there on gfx950; the product is exact either way.
+ * PG_PATH_NTT / PG_PATH_FFT force the transform (mul_ntt / mul_fft
+ * semantics, including mul_ntt's fall back to the FFT above 2^24 limbs or
+ * 2^26 points, mul_ntt.cpp:497-506). */
 enum { PG_PATH_AUTO = 0, PG_PATH_NTT = 1, PG_PATH_FFT = 2, PG_PATH_NTT_REF = 3 };
 /* PG_PATH_NTT runs the transform in the library's fastest digit mode (base
  * 2^32 digits over all three of preclib's primes); PG_PATH_NTT_REF runs

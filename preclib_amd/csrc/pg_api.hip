@@ -456,7 +456,8 @@ int pg_timing_read(pg_stage_time *out, int cap) {
 
 namespace pg {
 // Size-based routing above 192 limbs, following mul_into (src/mul_basic.cpp:
-// 259-269) and mul_ntt's fallbacks (src/mul_ntt.cpp:497-506).
+// 259-269) with the NTT for every transform-sized product (DISPATCH_* in
+// pg_kernels.h), and mul_ntt's fallbacks (src/mul_ntt.cpp:497-506).
 int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp, size_t bn,
                uint64_t btop, int path, double *fft_max_round) {
     if (fft_max_round) *fft_max_round = 0.0;
@@ -465,7 +466,7 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
     // Below the transform threshold mul_ntt / mul_fft fall back to mul_basic
     // (mul_ntt.cpp:496, mul_fft.cpp:388); mul_into takes the schoolbook for a
     // side <= 24 limbs and the blocked multiply for hi > 2 lo.
-    const bool small = hi <= 192 || (path == PG_PATH_AUTO && lo <= 24);
+    const bool small = hi <= DISPATCH_SMALL_HI || (path == PG_PATH_AUTO && lo <= DISPATCH_SMALL_SIDE);
     if (small || path == PG_PATH_AUTO) {
         if (small || (!sqr && hi > 2 * lo)) {
             // small leaf / mul_unbalanced_into (mul_basic.cpp:207-236) as a batch of one
@@ -484,7 +485,7 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
             // blocks are only reused by later work on the same stream
             return st;
         }
-        path = (!sqr && hi > 6144) ? PG_PATH_NTT : PG_PATH_FFT;
+        path = PG_PATH_NTT;
     }
     if (path == PG_PATH_NTT || path == PG_PATH_NTT_REF) {
         int st = ntt_mul_device(rp, ap, an, atop, bp, bn, btop, path == PG_PATH_NTT_REF ? MODE_REF : MODE_WIDE);
@@ -492,8 +493,8 @@ int mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const
         path = PG_PATH_FFT;   // beyond the NTT caps the reference multiplies with mul_fft
     }
     const int st = fft_mul_device(rp, ap, an, atop, bp, bn, btop, fft_max_round);
-    // precn_sqr sends every square to mul_fft (mul_basic.cpp:203); past the
-    // FFT's 2^24-point table the same exact product comes from the NTT.
+    // a square requested on the FFT past its 2^24-point table: the same exact
+    // product from the NTT
     if (st == PG_ERANGE && sqr) return ntt_mul_device(rp, ap, an, atop, bp, bn, btop, MODE_WIDE);
     return st;
 }

@@ -29,7 +29,6 @@ int launch_carry_nodes(size_t nnodes, double bytes, const CarryNode *dnodes, con
 
 namespace {
 
-constexpr size_t KARA_T = 24, FFT_T = 192, NTT_T = 6144;   // mul_basic.cpp:3-5
 constexpr size_t NTT_MAX_LIMBS = (size_t)1 << 24;           // mul_ntt.cpp:27
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -93,16 +92,12 @@ int Batch::plan(const uint64_t *x, size_t xn, const uint64_t *y, size_t yn, int 
         kind = 2;
     } else if (path == PG_PATH_FFT) {
         kind = 3;
-    } else if (lo <= KARA_T || hi <= FFT_T) {
+    } else if (lo <= DISPATCH_SMALL_SIDE || hi <= DISPATCH_SMALL_HI) {
         kind = 0;
-    } else if (sqr) {
-        kind = 3;   // precn_sqr: mul_fft for every size above 192 (mul_basic.cpp:203, 244-246)
-    } else if (hi > 2 * lo) {
+    } else if (!sqr && hi > 2 * lo) {
         kind = 1;
-    } else if (hi > NTT_T) {
-        kind = 2;
     } else {
-        kind = 3;
+        kind = 2;   // the NTT for every transform-sized product (DISPATCH_*)
     }
     if (kind == 2) {
         const size_t n16 = 4 * (xn + yn);   // digit count with full top limbs

@@ -111,6 +111,15 @@ enum { MODE_REF = 0, MODE_WIDE = 1 };
 // with a tile's operands adjacent, and the same with primes last to first.
 enum { COLF_OP_PRIME = 0, COLF_TILE_PRIME = 1, COLF_TILE_OP = 2, COLF_TILE_OP_REV = 3, ROW_PRIME_REV = 4 };
 constexpr int mode_primes(int mode) { return mode == MODE_WIDE ? 3 : 2; }
+// Product dispatch (mul_into, mul_basic.cpp:3-5, 238-284): schoolbook when a
+// side is <= 24 limbs or hi <= 192, blocked when hi > 2 lo, else a transform.
+// preclib's CPU rule sends hi in 193..6144 (and every square) to mul_fft and
+// larger products to mul_ntt.  On gfx950 the NTT takes every transform-sized
+// product (squares too) within its caps, and the FP64 FFT runs on request
+// (PG_PATH_FFT) and beyond the NTT caps: single products of 200..6144 limbs
+// measured 16-28 us on the NTT against 41-61 us on the FFT, and batches of
+// such products run 25% faster.  The product is exact on either path.
+constexpr size_t DISPATCH_SMALL_SIDE = 24, DISPATCH_SMALL_HI = 192;
 // Index of the prime whose transforms run on lazy residues in [0, 2p): the
 // set's first prime, 469762049 = 7 * 2^26 + 1, the one below 2^30 (the host
 // runtime checks this against its prime table).

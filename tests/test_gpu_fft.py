@@ -39,10 +39,14 @@ def test_fft_vs_oracle(pg, oracle, shape, dist):
 @pytest.mark.parametrize("dist", ["uniform", "ones", "sparse"])
 def test_fft_4096_golden(pg, golden, dist):
     a, b = W.pair(4096, 4096, dist)
-    got, err = pg.mul(a, b, want_round=True)        # dispatcher: 4096 <= 6144 -> FFT
+    got, err = pg.mul(a, b, path=pg.PATH_FFT, want_round=True)   # preclib's FFT-range shape
     e = next(x for x in golden["workload"] if x["an"] == 4096 and x["dist"] == dist)
     assert digest(got) == e["sha256"]
+    assert 0.0 <= err < 0.125
     print("max rounding distance", dist, err)
+    # the dispatcher sends the same shape to the NTT on the GPU (no rounding)
+    auto, aerr = pg.mul(a, b, want_round=True)
+    assert digest(auto) == e["sha256"] and aerr == 0.0
 
 
 def test_fft_torture_golden(pg, golden):

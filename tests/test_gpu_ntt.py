@@ -107,3 +107,16 @@ def test_plan_variants(pg, oracle, golden, v):
                 assert hashlib.sha256(r.astype("<u8").tobytes()).hexdigest() == c["sha256"], (v, c["an"], c["dist"])
     finally:
         pg.set_ntt_plans(saved)
+
+
+@pytest.mark.parametrize("n", [193, 1000, 4096, 6144, 20000])
+def test_auto_dispatch_squares_and_mid_sizes(pg, oracle, n):
+    """The dispatcher sends squares and balanced products above 192 limbs to
+    the NTT (no rounding distance reported); each equals the oracle's
+    mul_into, including the aliased square precn_sqr would take."""
+    a, b = W.pair(n, n, "uniform", k=n)
+    print("case", n, flush=True)
+    r, err = pg.mul(a, b, want_round=True)
+    assert err == 0.0 and np.array_equal(r, oracle.mul_into(a, b))
+    s, serr = pg.mul(a, a, want_round=True)
+    assert serr == 0.0 and np.array_equal(s, oracle.mul_into(a, a, sqr=True))
