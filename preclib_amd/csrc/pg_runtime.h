@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <map>
 #include <memory>
+#include <new>
 #include <utility>
 #include <mutex>
 #include <string>
@@ -11,17 +12,23 @@
 
 namespace pg {
 
-// Allocator whose value-construction leaves the bytes as they are: resizing a
-// HostImg does not zero it (every byte that matters is written by its user).
+// Page-locked host memory (hipHostMalloc) for descriptor images: an upload
+// from it is a plain DMA that returns at once, where one from pageable memory
+// is staged and can hold the host until the stream drains.  Value
+// construction leaves the bytes as they are, so resizing an image does not
+// zero it (every byte that matters is written by its user).
 template <class T>
-struct NoInitAlloc : std::allocator<T> {
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
     template <class U>
-    struct rebind {
-        using other = NoInitAlloc<U>;
-    };
-    NoInitAlloc() = default;
-    template <class U>
-    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    PinnedAlloc(const PinnedAlloc<U> &) noexcept {}
+    T *allocate(size_t n) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess || !p) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t) noexcept { (void)hipHostFree(p); }
     template <class U>
     void construct(U *p) noexcept {
         ::new ((void *)p) U;
@@ -30,8 +37,12 @@ struct NoInitAlloc : std::allocator<T> {
     void construct(U *p, A &&...a) {
         ::new ((void *)p) U(std::forward<A>(a)...);
     }
+    template <class U>
+    bool operator==(const PinnedAlloc<U> &) const noexcept { return true; }
+    template <class U>
+    bool operator!=(const PinnedAlloc<U> &) const noexcept { return false; }
 };
-using HostImg = std::vector<char, NoInitAlloc<char>>;
+using HostImg = std::vector<char, PinnedAlloc<char>>;
 
 // NTT primes and generators (src/mul_ntt.cpp:29-34).
 constexpr uint32_t PRIME[3] = {469762049u, 1811939329u, 2013265921u};
