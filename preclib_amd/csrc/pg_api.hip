@@ -26,19 +26,25 @@ struct pg_buf {
 
 extern "C" {
 
-int pg_init(int device, size_t hbm_budget) { return ctx().init(device, hbm_budget); }
+int pg_init(int device, size_t hbm_budget) try {
+    return ctx().init(device, hbm_budget);
+} catch (...) {
+    return pg::exception_status();
+}
 
 void pg_shutdown(void) { ctx().shutdown(); }
 
 const char *pg_last_error(void) { return pg::last_error(); }
 
-int pg_sync(void) {
+int pg_sync(void) try {
     LOCK;
     NEED_INIT;
     return ctx().sync();
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_buf_alloc(pg_buf **h, size_t limbs) {
+int pg_buf_alloc(pg_buf **h, size_t limbs) try {
     LOCK;
     NEED_INIT;
     if (!h || !limbs) return set_error(PG_EINVAL, "pg_buf_alloc: bad argument");
@@ -46,20 +52,24 @@ int pg_buf_alloc(pg_buf **h, size_t limbs) {
     if (!p) return set_error(PG_ENOMEM, "pg_buf_alloc: device budget exhausted");
     *h = new pg_buf{(uint64_t *)p, limbs};
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_buf_free(pg_buf *h) {
+int pg_buf_free(pg_buf *h) try {
     LOCK;
     if (!h) return PG_OK;
     ctx().pool.put(h->d);
     delete h;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
 size_t pg_buf_capacity(const pg_buf *h) { return h ? h->cap : 0; }
 uint64_t *pg_buf_ptr(pg_buf *h) { return h ? h->d : nullptr; }
 
-int pg_buf_upload(pg_buf *h, const uint64_t *src, size_t limbs) {
+int pg_buf_upload(pg_buf *h, const uint64_t *src, size_t limbs) try {
     LOCK;
     NEED_INIT;
     if (!h || (!src && limbs) || limbs > h->cap) return set_error(PG_EINVAL, "pg_buf_upload: bad argument");
@@ -67,9 +77,11 @@ int pg_buf_upload(pg_buf *h, const uint64_t *src, size_t limbs) {
         hipStreamSynchronize(ctx().stream) != hipSuccess)
         return set_error(PG_EDEVICE, "pg_buf_upload: copy failed");
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs) {
+int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs) try {
     LOCK;
     NEED_INIT;
     if (!h || (!dst && limbs) || limbs > h->cap) return set_error(PG_EINVAL, "pg_buf_download: bad argument");
@@ -77,6 +89,8 @@ int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs) {
         hipStreamSynchronize(ctx().stream) != hipSuccess)
         return set_error(PG_EDEVICE, "pg_buf_download: copy failed");
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
 static int check_operands(size_t an, size_t bn, size_t r_cap) {
@@ -86,7 +100,7 @@ static int check_operands(size_t an, size_t bn, size_t r_cap) {
 }
 
 int pg_mul(uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a, size_t an, const uint64_t *b,
-           size_t bn, int path, double *fft_max_round) {
+           size_t bn, int path, double *fft_max_round) try {
     LOCK;
     NEED_INIT;
     if (!an || !bn) {   // a zero operand gives the empty product (mul_basic.cpp:239-242)
@@ -127,10 +141,12 @@ int pg_mul(uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a, size_t a
     while (len && r[len - 1] == 0) --len;
     if (r_len) *r_len = len;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
 int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an, const pg_buf *b, size_t bn, int path,
-               double *fft_max_round) {
+               double *fft_max_round) try {
     LOCK;
     NEED_INIT;
     if (!r || !a || !b) return set_error(PG_EINVAL, "pg_mul_dev: null handle");
@@ -151,13 +167,15 @@ int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an, const pg_bu
         *r_len = top ? an + bn : an + bn - 1;
     }
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
 struct pg_batch {
     pg::Batch b;
 };
 
-int pg_batch_plan(pg_batch **out, const pg_job *jobs, size_t n) {
+int pg_batch_plan(pg_batch **out, const pg_job *jobs, size_t n) try {
     LOCK;
     NEED_INIT;
     if (!out || (!jobs && n)) return set_error(PG_EINVAL, "pg_batch_plan: bad argument");
@@ -169,16 +187,20 @@ int pg_batch_plan(pg_batch **out, const pg_job *jobs, size_t n) {
     }
     *out = h;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_batch_run(pg_batch *h) {
+int pg_batch_run(pg_batch *h) try {
     LOCK;
     NEED_INIT;
     if (!h) return set_error(PG_EINVAL, "pg_batch_run: null handle");
     return h->b.run();
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_batch_errors(pg_batch *h, double *err, size_t n) {
+int pg_batch_errors(pg_batch *h, double *err, size_t n) try {
     LOCK;
     NEED_INIT;
     if (!h || (!err && n)) return set_error(PG_EINVAL, "pg_batch_errors: bad argument");
@@ -187,18 +209,22 @@ int pg_batch_errors(pg_batch *h, double *err, size_t n) {
     if (st) return st;
     for (size_t i = 0; i < n && i < e.size(); ++i) err[i] = e[i];
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_batch_free(pg_batch *h) {
+int pg_batch_free(pg_batch *h) try {
     LOCK;
     if (h) {
         if (ctx().stream) hipStreamSynchronize(ctx().stream);
         delete h;
     }
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_mul_batch(pg_job *jobs, size_t n) {
+int pg_mul_batch(pg_job *jobs, size_t n) try {
     LOCK;
     NEED_INIT;
     if (!jobs && n) return set_error(PG_EINVAL, "pg_mul_batch: null jobs");
@@ -250,6 +276,8 @@ int pg_mul_batch(pg_job *jobs, size_t n) {
         jobs[i].fft_max_round = i < errs.size() ? errs[i] : 0.0;
     }
     return st;
+} catch (...) {
+    return pg::exception_status();
 }
 
 // ---- Newton division (div_newton.cpp) -------------------------------------------
@@ -271,7 +299,7 @@ static int finish_out(const DBuf &src, size_t len, uint64_t *dst, size_t cap, si
 }
 
 int pg_div_dev(pg_buf *q, size_t q_cap, size_t *q_len, pg_buf *r, size_t r_cap, size_t *r_len, const pg_buf *a,
-               size_t an, const pg_buf *b, size_t bn) {
+               size_t an, const pg_buf *b, size_t bn) try {
     LOCK;
     NEED_INIT;
     if (!a || !b || an > a->cap || bn > b->cap || (q && q_cap > q->cap) || (r && r_cap > r->cap))
@@ -285,10 +313,12 @@ int pg_div_dev(pg_buf *q, size_t q_cap, size_t *q_len, pg_buf *r, size_t r_cap, 
     if ((st = div_mulinv(dq, ql, r ? &dr : nullptr, &rl, a->d, at, b->d, bt))) return st;
     if ((st = finish_out(dq, ql, q ? q->d : nullptr, q_cap, q_len, true))) return st;
     return finish_out(dr, rl, r ? r->d : nullptr, r_cap, r_len, true);
+} catch (...) {
+    return pg::exception_status();
 }
 
 int pg_div(uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a,
-           size_t an, const uint64_t *b, size_t bn) {
+           size_t an, const uint64_t *b, size_t bn) try {
     LOCK;
     NEED_INIT;
     if ((an && !a) || (bn && !b)) return set_error(PG_EINVAL, "pg_div: bad argument");
@@ -304,9 +334,11 @@ int pg_div(uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *r, size_t r_cap, 
     if (st) return st;
     if ((st = finish_out(dq, ql, q, q_cap, q_len, false))) return st;
     return finish_out(dr, rl, r, r_cap, r_len, false);
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_recip_dev(pg_buf *x, size_t x_cap, size_t *x_len, const pg_buf *b, size_t bn, size_t bits) {
+int pg_recip_dev(pg_buf *x, size_t x_cap, size_t *x_len, const pg_buf *b, size_t bn, size_t bits) try {
     LOCK;
     NEED_INIT;
     if (!x || !b || bn > b->cap || x_cap > x->cap) return set_error(PG_EINVAL, "pg_recip_dev: bad argument");
@@ -316,9 +348,11 @@ int pg_recip_dev(pg_buf *x, size_t x_cap, size_t *x_len, const pg_buf *b, size_t
     DBuf dx;
     if ((st = recip_exact(dx, xl, b->d, bt, bits))) return st;
     return finish_out(dx, xl, x->d, x_cap, x_len, true);
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t bn, size_t bits) {
+int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t bn, size_t bits) try {
     LOCK;
     NEED_INIT;
     if (bn && !b) return set_error(PG_EINVAL, "pg_recip: bad argument");
@@ -331,11 +365,13 @@ int pg_recip(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *b, size_t
     int st = recip_exact(dx, xl, db.d, bn, bits);
     if (st) return st;
     return finish_out(dx, xl, x, x_cap, x_len, false);
+} catch (...) {
+    return pg::exception_status();
 }
 
 // ---- square root, decimal conversion, binary splitting, pi -------------------------
 
-int pg_sqrt(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *a, size_t an) {
+int pg_sqrt(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *a, size_t an) try {
     LOCK;
     NEED_INIT;
     if (an && !a) return set_error(PG_EINVAL, "pg_sqrt: bad argument");
@@ -348,6 +384,8 @@ int pg_sqrt(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *a, size_t 
     int st = isqrt(dx, xl, da.d, an);
     if (st) return st;
     return finish_out(dx, xl, x, x_cap, x_len, false);
+} catch (...) {
+    return pg::exception_status();
 }
 
 static int put_string(const std::string &s, char *out, size_t cap, size_t *len) {
@@ -358,7 +396,7 @@ static int put_string(const std::string &s, char *out, size_t cap, size_t *len) 
     return PG_OK;
 }
 
-int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t an) {
+int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t an) try {
     LOCK;
     NEED_INIT;
     if (an && !a) return set_error(PG_EINVAL, "pg_to_decimal: bad argument");
@@ -371,10 +409,12 @@ int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t 
     int st = to_decimal(da.d, an, s);
     if (st) return st;
     return put_string(s, out, cap, len);
+} catch (...) {
+    return pg::exception_status();
 }
 
 int pg_chud_split(uint64_t *p, size_t p_cap, size_t *p_len, uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *t,
-                  size_t t_cap, size_t *t_len, size_t nterms) {
+                  size_t t_cap, size_t *t_len, size_t nterms) try {
     LOCK;
     NEED_INIT;
     std::vector<DBuf> keep;
@@ -385,57 +425,71 @@ int pg_chud_split(uint64_t *p, size_t p_cap, size_t *p_len, uint64_t *q, size_t 
     if ((st = finish_out(P, lp, p, p_cap, p_len, false))) return st;
     if ((st = finish_out(Q, lq, q, q_cap, q_len, false))) return st;
     return finish_out(T, lt, t, t_cap, t_len, false);
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms) {
+int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms) try {
     LOCK;
     NEED_INIT;
     std::string s;
     int st = pi_digits(digits, s, phase_ms);
     if (st) return st;
     return put_string(s, out, cap, len);
+} catch (...) {
+    return pg::exception_status();
 }
 
 // ---- plan autotuner ------------------------------------------------------------------
 
-int pg_ntt_autotune(int logn_lo, int logn_hi, int reps, double *ms, int *chosen) {
+int pg_ntt_autotune(int logn_lo, int logn_hi, int reps, double *ms, int *chosen) try {
     LOCK;
     NEED_INIT;
     if (!ms || !chosen) return set_error(PG_EINVAL, "pg_ntt_autotune: null");
     return autotune_ntt(logn_lo, logn_hi, reps, ms, chosen);
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_ntt_plan_get(int *variants, int n) {
+int pg_ntt_plan_get(int *variants, int n) try {
     LOCK;
     NEED_INIT;
     for (int i = 0; i < n; ++i) variants[i] = i <= NTT_MAX_LOG ? ctx().ntt_variant_tab[i] : 0;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_ntt_plan_set(const int *variants, int n) {
+int pg_ntt_plan_set(const int *variants, int n) try {
     LOCK;
     NEED_INIT;
     for (int i = 0; i < n && i <= NTT_MAX_LOG; ++i)
         if (variants[i] < 0 || variants[i] >= NTT_VARIANTS) return set_error(PG_EINVAL, "pg_ntt_plan_set: variant");
     for (int i = 0; i < n && i <= NTT_MAX_LOG; ++i) ctx().ntt_variant_tab[i] = i >= NTT_VARIANT_MIN_LOG ? variants[i] : 0;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_timing_enable(int on) {
+int pg_timing_enable(int on) try {
     LOCK;
     ctx().timing = on != 0;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_timing_reset(void) {
+int pg_timing_reset(void) try {
     LOCK;
     if (ctx().stream) hipStreamSynchronize(ctx().stream);
     ctx().collect_timing();
     ctx().stats.clear();
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_timing_read(pg_stage_time *out, int cap) {
+int pg_timing_read(pg_stage_time *out, int cap) try {
     LOCK;
     if (ctx().stream) hipStreamSynchronize(ctx().stream);
     ctx().collect_timing();
@@ -450,6 +504,8 @@ int pg_timing_read(pg_stage_time *out, int cap) {
         ++i;
     }
     return i;
+} catch (...) {
+    return pg::exception_status();
 }
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <map>
+#include <cstdlib>
 #include <memory>
 #include <new>
 #include <utility>
@@ -23,12 +24,26 @@ struct PinnedAlloc {
     PinnedAlloc() = default;
     template <class U>
     PinnedAlloc(const PinnedAlloc<U> &) noexcept {}
+    // A 16-byte header records where the block came from: page-locked memory,
+    // or (when the runtime refuses more of it) the ordinary heap.
     T *allocate(size_t n) {
+        const size_t bytes = n * sizeof(T) + 16;
         void *p = nullptr;
-        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess || !p) throw std::bad_alloc();
-        return static_cast<T *>(p);
+        uint64_t from = 1;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+            (void)hipGetLastError();   // not a device error: clear it
+            from = 2;
+            p = std::malloc(bytes);
+            if (!p) throw std::bad_alloc();
+        }
+        *static_cast<uint64_t *>(p) = from;
+        return reinterpret_cast<T *>(static_cast<char *>(p) + 16);
     }
-    void deallocate(T *p, size_t) noexcept { (void)hipHostFree(p); }
+    void deallocate(T *q, size_t) noexcept {
+        char *p = reinterpret_cast<char *>(q) - 16;
+        if (*reinterpret_cast<uint64_t *>(p) == 1) (void)hipHostFree(p);
+        else std::free(p);
+    }
     template <class U>
     void construct(U *p) noexcept {
         ::new ((void *)p) U;
@@ -170,6 +185,9 @@ private:
 
 Context &ctx();
 int set_error(int code, const std::string &msg);
+// Status of the exception being handled: the C ABI lets none escape (host
+// allocation failure -> PG_ENOMEM, anything else -> PG_EINTERNAL).
+int exception_status();
 
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \

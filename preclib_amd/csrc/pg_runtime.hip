@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include "pgpu.h"
 
 namespace pg {
@@ -17,6 +18,18 @@ int set_error(int code, const std::string &msg) {
     return code;
 }
 const char *last_error() { return g_err.c_str(); }
+
+int exception_status() {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return set_error(PG_ENOMEM, "host memory exhausted");
+    } catch (const std::exception &e) {
+        return set_error(PG_EINTERNAL, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return set_error(PG_EINTERNAL, "internal error");
+    }
+}
 
 
 uint32_t powmod(uint32_t b, uint64_t e, uint32_t p) {

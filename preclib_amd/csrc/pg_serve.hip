@@ -19,6 +19,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <new>
 #include <thread>
 
 #include "pg_batch.h"
@@ -325,23 +326,32 @@ int Server::finish(Req &q) {
 void Server::round() {
     Context &C = ctx();
     std::lock_guard<std::recursive_mutex> lk(C.mu);
-    std::vector<CarryNode> pre, post;
     std::vector<pg_job> prod;
-    for (auto &q : active)
-        if (q->stage != S_DONE) plan(*q, pre, prod, post);
-    int st = lin_run(pre.data(), pre.size(), "serve");
-    if (!st && !prod.empty()) {
-        Batch b;
-        b.use_graph = false;
-        st = b.build(prod.data(), prod.size());
-        if (!st) st = b.run();
+    int st = PG_OK;
+    try {
+        std::vector<CarryNode> pre, post;
+        for (auto &q : active)
+            if (q->stage != S_DONE) plan(*q, pre, prod, post);
+        st = lin_run(pre.data(), pre.size(), "serve");
+        if (!st && !prod.empty()) {
+            Batch b;
+            b.use_graph = false;
+            st = b.build(prod.data(), prod.size());
+            if (!st) st = b.run();
+        }
+        if (!st) st = lin_run(post.data(), post.size(), "serve");
+        for (auto &q : active)
+            if (!st && q->read_flag && q->stage != S_DONE &&
+                hipMemcpyAsync(&q->flag_host, q->flag.d, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess)
+                st = set_error(PG_EDEVICE, "server: flag read");
+    } catch (...) {
+        // the round's requests fail; work already queued finishes first
+        st = exception_status();
     }
-    if (!st) st = lin_run(post.data(), post.size(), "serve");
-    for (auto &q : active)
-        if (!st && q->read_flag && q->stage != S_DONE &&
-            hipMemcpyAsync(&q->flag_host, q->flag.d, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess)
-            st = set_error(PG_EDEVICE, "server: flag read");
-    if (!st) st = C.sync();
+    {
+        const int sst = C.sync();
+        if (!st) st = sst;
+    }
     stats.rounds++;
     stats.products += prod.size();
     for (auto &q : active) {
@@ -396,8 +406,27 @@ void Server::run() {
                 int st;
                 {
                     std::lock_guard<std::recursive_mutex> lk(ctx().mu);
-                    st = admit(u, q);
+                    try {
+                        st = admit(u, q);
+                    } catch (...) {
+                        st = exception_status();
+                    }
                 }
+                if (!q) {   // admit failed before its record existed
+                    q.reset(new (std::nothrow) Req);
+                    if (!q) {   // no record at all: the request completes failed now
+                        used -= fp;
+                        stats.completed++;
+                        outstanding--;
+                        u->t_done = now_s();
+                        u->status = st ? st : PG_ENOMEM;
+                        cv_done.notify_all();
+                        continue;
+                    }
+                    q->user = u;
+                    q->op = u->op;
+                }
+                q->fp = fp;   // what admission added to `used`
                 if (st) fail(*q, st);
                 active.push_back(std::move(q));
                 stats.max_active = std::max<uint64_t>(stats.max_active, active.size());
@@ -419,7 +448,7 @@ extern "C" {
 
 double pg_server_clock(void) { return now_s(); }
 
-int pg_server_start(pg_server **out, size_t hbm_budget, size_t max_active) {
+int pg_server_start(pg_server **out, size_t hbm_budget, size_t max_active) try {
     if (!out) return set_error(PG_EINVAL, "pg_server_start: null");
     if (!ctx().stream) return set_error(PG_ENOINIT, "pg_init not called");
     pg_server *h = new pg_server;
@@ -428,9 +457,11 @@ int pg_server_start(pg_server **out, size_t hbm_budget, size_t max_active) {
     h->s.worker = std::thread([h] { h->s.run(); });
     *out = h;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_server_submit(pg_server *h, pg_request *req) {
+int pg_server_submit(pg_server *h, pg_request *req) try {
     if (!h || !req || (req->op != PG_OP_MUL && req->op != PG_OP_DIV) || (req->an && !req->a) || (req->bn && !req->b) ||
         !req->r)
         return set_error(PG_EINVAL, "pg_server_submit: bad request");
@@ -455,30 +486,38 @@ int pg_server_submit(pg_server *h, pg_request *req) {
     }
     h->s.cv_work.notify_one();
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_server_wait(pg_server *h, pg_request *req) {
+int pg_server_wait(pg_server *h, pg_request *req) try {
     if (!h || !req) return set_error(PG_EINVAL, "pg_server_wait: null");
     std::unique_lock<std::mutex> g(h->s.mu);
     h->s.cv_done.wait(g, [&] { return req->status != 1; });
     return req->status;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_server_drain(pg_server *h) {
+int pg_server_drain(pg_server *h) try {
     if (!h) return set_error(PG_EINVAL, "pg_server_drain: null");
     std::unique_lock<std::mutex> g(h->s.mu);
     h->s.cv_done.wait(g, [&] { return h->s.outstanding == 0; });
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_server_read_stats(pg_server *h, pg_server_stats *st) {
+int pg_server_read_stats(pg_server *h, pg_server_stats *st) try {
     if (!h || !st) return set_error(PG_EINVAL, "pg_server_read_stats: null");
     std::lock_guard<std::mutex> g(h->s.mu);
     *st = h->s.stats;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
-int pg_server_stop(pg_server *h) {
+int pg_server_stop(pg_server *h) try {
     if (!h) return PG_OK;
     {
         std::lock_guard<std::mutex> g(h->s.mu);
@@ -488,6 +527,8 @@ int pg_server_stop(pg_server *h) {
     if (h->s.worker.joinable()) h->s.worker.join();
     delete h;
     return PG_OK;
+} catch (...) {
+    return pg::exception_status();
 }
 
 }  // extern "C"
