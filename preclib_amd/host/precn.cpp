@@ -4,8 +4,8 @@
 // moved-from objects hold a == nullptr).  Products, divisions, square roots
 // and base-10 conversion go through pg_mul / pg_div / pg_recip / pg_sqrt /
 // pg_to_decimal; the dispatcher rules of mul_into (mul_basic.cpp:238-276)
-// are applied by the library (PG_PATH_AUTO) so the GPU path matches the
-// reference's routing.  Additions, subtractions, shifts and comparisons are
+// are applied by the library (PG_PATH_AUTO): the reference's structure, with
+// the NTT where preclib's CPU thresholds pick mul_fft (pgpu.h).  Additions, subtractions, shifts and comparisons are
 // the linear host steps of src/add.cpp, src/sub.cpp and
 // src/compare_shift.cpp.
 #include "precn.hpp"
@@ -173,8 +173,8 @@ precn_t operator*(const precn_t &a, const precn_t &b) {   // mul_basic.cpp:278-2
 }
 
 precn_t precn_sqr(const precn_t &a) {
-    // mul_basic.cpp:193-205: the library sees one operand (same pointer) and
-    // routes the square as precn_sqr does (FFT above 192 limbs)
+    // mul_basic.cpp:193-205: the library sees one operand (same pointer);
+    // above 192 limbs it squares on the NTT (preclib: mul_fft)
     precn_t r;
     gpu_mul(r, a, a, PG_PATH_AUTO);
     return r;
