@@ -562,8 +562,8 @@ __device__ __forceinline__ void stage_prime(uint32_t *lds, const NttLaunch &L, c
         uint32_t w = twiddle_mont(L.ilo[pr], L.ihi[pr], (C * f0) & NMASK, L.lobits, c);
         // omega_n^(-C * 2^D) in Montgomery form, from the forward tables
         const uint32_t st = twiddle_mont(L.glo[pr], L.ghi[pr], (0u - (C << TW::D)) & NMASK, L.lobits, c);
-        // residues in groups of 4 loads in flight
-        constexpr int G = TW::E < 4 ? TW::E : 4;
+        // residues in groups of 8 loads in flight (E is a power of two)
+        constexpr int G = TW::E < 8 ? TW::E : 8;
 #pragma unroll 1
         for (int k0 = 0; k0 < TW::E; k0 += G) {
             uint32_t x[G];
