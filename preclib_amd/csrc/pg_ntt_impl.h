@@ -63,6 +63,16 @@ __device__ __forceinline__ int fresh_tid() {
     return t;
 }
 
+// Global-memory view of a device pointer read from a descriptor: loads
+// through it are global_load (not flat) instructions.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+// 16-byte vector of the compiler's own (loads through an address-space
+// pointer need a builtin vector type; HIP's uint4 is a class)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // ---- in-LDS radix passes --------------------------------------------------
 
 // HALFZ (first pass only): inputs q >= R/2, the upper half of the
@@ -344,7 +354,9 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
     const bool half = L1 > 0 && LR0 > 0 && (size_t)nl * DPL <= ((size_t)1 << (LOGN - 1));
     {
         const PrimeK &c = L.pk[pr];
-        // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T)
+        // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T).  (Loads
+        // grouped ahead of their LDS writes measured slower here: at 64 VGPRs
+        // the kernel spills.)
         const int ne = (half ? N1 / 2 : N1) * LPR;
         for (int e = threadIdx.x; e < ne; e += NTT_THREADS) {
             const int j1 = e / LPR, m = e % LPR;
@@ -381,23 +393,63 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
 
 // ---- kernel 2: row pass (forward A, forward B, pointwise, inverse) -------------
 
-template <int LOGN, int MODE, int V>
-__device__ __forceinline__ void load_row(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int op, int pr,
-                                         int row) {
+// Operand rows op0 .. op0 + NOPS - 1 of the row kernel into lds + k * N2.
+// Every thread issues all its loads (NOPS * N2 / 4 / ROW_THREADS of 16
+// bytes) before the first LDS write, so a workgroup waits for memory once.
+template <int LOGN, int MODE, int V, int NOPS>
+__device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int op0, int pr,
+                                          int row) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
     if constexpr (L1 == 0) {
         // single-row plan: the row is the digit vector itself
         constexpr int DPL = 64 / mode_digit_bits(MODE);
-        const uint64_t *src = op ? D.b : D.a;
-        const size_t nl = op ? D.bn : D.an;
-        for (int m = threadIdx.x; m < N2 / DPL; m += ROW_THREADS)
-            put_digits<MODE>(lds, DPL * m, (size_t)m < nl ? src[m] : 0, L.crt.red[pr], L.pk[pr].p);
-    } else {
-        const uint4 *in = (const uint4 *)(D.spec + ((size_t)(op * mode_primes(MODE) + pr) << LOGN) +
+#pragma unroll
+        for (int k = 0; k < NOPS; ++k) {
+            const int op = op0 + k;
+            const uint64_t *src = op ? D.b : D.a;
+            const size_t nl = op ? D.bn : D.an;
+            for (int m = threadIdx.x; m < N2 / DPL; m += ROW_THREADS)
+                put_digits<MODE>(lds + k * N2, DPL * m, (size_t)m < nl ? src[m] : 0, L.crt.red[pr], L.pk[pr].p);
+        }
+    } else if constexpr (NOPS == 1 && (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS > 4) {
+        // the longest single rows: one load at a time (that kernel has no
+        // registers to spare)
+        const uint4 *in = (const uint4 *)(D.spec + ((size_t)(op0 * mode_primes(MODE) + pr) << LOGN) +
                                           ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
             const uint4 v = in[m];
             lds[lp(4 * m)] = v.x; lds[lp(4 * m + 1)] = v.y; lds[lp(4 * m + 2)] = v.z; lds[lp(4 * m + 3)] = v.w;
+        }
+    } else {
+        // loads of 16 bytes, up to 4 per operand row in flight per thread
+        // (E4 is a power of two)
+        constexpr int E4 = (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS;
+        constexpr int GR = E4 < 4 ? E4 : 4;
+#pragma unroll
+        for (int i0 = 0; i0 < E4; i0 += GR) {
+            u32x4 v[NOPS][GR];
+#pragma unroll
+            for (int k = 0; k < NOPS; ++k) {
+                const auto in = gptr((const u32x4 *)(D.spec + ((size_t)((op0 + k) * mode_primes(MODE) + pr) << LOGN) +
+                                                     ((size_t)row << L2)));
+#pragma unroll
+                for (int i = 0; i < GR; ++i) {
+                    const int m = threadIdx.x + (i0 + i) * ROW_THREADS;
+                    if (m < N2 / 4) v[k][i] = in[m];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NOPS; ++k) {
+                uint32_t *l = lds + k * N2;
+#pragma unroll
+                for (int i = 0; i < GR; ++i) {
+                    const int m = threadIdx.x + (i0 + i) * ROW_THREADS;
+                    if (m < N2 / 4) {
+                        l[lp(4 * m)] = v[k][i].x; l[lp(4 * m + 1)] = v[k][i].y;
+                        l[lp(4 * m + 2)] = v[k][i].z; l[lp(4 * m + 3)] = v[k][i].w;
+                    }
+                }
+            }
         }
     }
 }
@@ -421,8 +473,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
         // last pass, the pointwise product and the inverse's first pass run in
         // registers, and the inverse finishes in B's tile.
         uint32_t *lda = lds, *ldb = lds + N2;
-        load_row<LOGN, MODE, V>(lda, L, D, 0, pr, row);
-        load_row<LOGN, MODE, V>(ldb, L, D, 1, pr, row);
+        load_rows<LOGN, MODE, V, 2>(lda, L, D, 0, pr, row);   // A at lda, B at lda + N2 = ldb
         lds_sync();
         fwd2_from<L2, 0, ROW_THREADS, L2, 0, RBV, NPS - 1, LZ>(lda, ldb, L.row_tw[pr], c, threadIdx.x);
 #pragma unroll
@@ -456,7 +507,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
     constexpr int E4 = (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS;
     // larger rows would spill; radix-8 rows at 64 VGPRs rely on occupancy instead
     constexpr bool PF = plan_l1(LOGN, V) > 0 && E4 <= 4 && row_waves(LOGN, V) < 8;
-    load_row<LOGN, MODE, V>(lds, L, D, 0, pr, row);
+    load_rows<LOGN, MODE, V, 1>(lds, L, D, 0, pr, row);
     // operand B's spectrum row is fetched now and lands during A's transform
     uint4 pb[PF ? E4 : 1];
     const uint4 *inb = (const uint4 *)(D.spec + ((size_t)(mode_primes(MODE) + pr) << LOGN) + ((size_t)row << L2));
@@ -489,7 +540,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
             }
         }
     } else {
-        load_row<LOGN, MODE, V>(lds, L, D, 1, pr, row);
+        load_rows<LOGN, MODE, V, 1>(lds, L, D, 1, pr, row);
     }
     lds_sync();
     lds_forward<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_tw[pr], c);
