@@ -355,8 +355,8 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
     {
         const PrimeK &c = L.pk[pr];
         // digits of the tile: row j1 holds digits j1*N2 + c0 + [0, T).  (Loads
-        // grouped ahead of their LDS writes measured slower here: at 64 VGPRs
-        // the kernel spills.)
+        // grouped ahead of their LDS writes measured slower here: 32 waves per
+        // CU already hide the latency, and groups of 4 spill at 64 VGPRs.)
         const int ne = (half ? N1 / 2 : N1) * LPR;
         for (int e = threadIdx.x; e < ne; e += NTT_THREADS) {
             const int j1 = e / LPR, m = e % LPR;
