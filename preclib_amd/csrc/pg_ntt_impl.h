@@ -168,6 +168,113 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
     }
 }
 
+// Twiddles of one inverse pass for the columns a thread handles, loaded
+// into registers ahead of the pass (before the barrier that precedes it), so
+// the pass starts on its products instead of waiting for the loads.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT>
+struct PassTw {
+    static constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
+    static constexpr int NCOLS = 1 << (LOGN - LR + LOGT), IT = (NCOLS + NT - 1) / NT;
+    uint2 w[M > 1 ? IT : 1][M > 1 ? R : 1];
+    __device__ __forceinline__ void load(const uint2 *__restrict__ itw, int tid) {
+        if constexpr (M > 1) {
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int col = tid + it * NT;
+                if (col < NCOLS) {
+                    const uint2 *t = itw + ((col >> LOGT) & (M - 1));
+#pragma unroll
+                    for (int q = 1; q < R; ++q) w[it][q] = t[bitrev<R>(q) * M];
+                }
+            }
+        }
+    }
+};
+
+// pass_inv with the pass's twiddles already in registers.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool LZ>
+__device__ __forceinline__ void pass_inv_pre(uint32_t *lds, const PassTw<LR, LOGN, LOGT, LOGB, NT> &tw,
+                                             const PrimeK &c, int tid) {
+    using TW = PassTw<LR, LOGN, LOGT, LOGB, NT>;
+    constexpr int R = TW::R, LOGM = TW::LOGM, M = TW::M, T = TW::T, S = M * T;
+#pragma unroll
+    for (int it = 0; it < TW::IT; ++it) {
+        const int col = tid + it * NT;
+        if (col < TW::NCOLS) {
+            const int cc = col & (T - 1), r = col >> LOGT;
+            const int blk = r >> LOGM, j2 = r & (M - 1);
+            const int base = (((blk << LOGB) + j2) << LOGT) + cc;
+            uint32_t x[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) x[q] = lds[lp(base + bitrev<R>(q) * S)];
+            if constexpr (M > 1) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) x[q] = mul_shoup<LZ>(x[q], tw.w[it][q].x, tw.w[it][q].y, c);
+            }
+            dft_dit_inv<R, LZ>(x, c);
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds[lp(base + q * S)] = x[q];
+        }
+    }
+}
+
+// Block size (log2) of forward pass I of a 2^LOGN transform.
+template <int LOGN, int I, int RB>
+constexpr int pass_logb() {
+    int b = LOGN;
+    for (int k = 0; k < I; ++k) b -= split_lr(LOGN, k, RB);
+    return b;
+}
+template <int LOGN, int LOGT, int NT, int I, int RB>
+using PassTwOf = PassTw<split_lr(LOGN, I, RB), LOGN, LOGT, pass_logb<LOGN, I, RB>(), NT>;
+
+// Inverse passes I, I - 1, ..., 0 (each followed by a barrier), pass I with
+// its twiddles in `cur`; each next pass's twiddles are loaded before the
+// barrier that precedes it.
+template <int LOGN, int LOGT, int NT, int I, int RB, bool LZ>
+__device__ __forceinline__ void inv_pre_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid,
+                                             const PassTwOf<LOGN, LOGT, NT, I, RB> &cur) {
+    pass_inv_pre<split_lr(LOGN, I, RB), LOGN, LOGT, pass_logb<LOGN, I, RB>(), NT, LZ>(lds, cur, c, tid);
+    if constexpr (I > 0) {
+        PassTwOf<LOGN, LOGT, NT, I - 1, RB> nxt;
+        nxt.load(itw[I - 1], tid);
+        lds_sync();
+        inv_pre_from<LOGN, LOGT, NT, I - 1, RB, LZ>(lds, itw, c, tid, nxt);
+    } else {
+        lds_sync();
+    }
+}
+
+// Whether every inverse pass below STOP keeps at most 15 twiddle pairs per
+// thread (the register budget of the prefetch).
+template <int LOGN, int LOGT, int NT, int RB, int STOP>
+constexpr bool inv_pre_fits() {
+    for (int i = 0; i < STOP; ++i) {
+        const int lr = split_lr(LOGN, i, RB);
+        const int ncols = 1 << (LOGN - lr + LOGT);
+        if (lr <= 0 || ((ncols + NT - 1) / NT) * ((1 << lr) - 1) > 15) return false;
+    }
+    return true;
+}
+
+// lds_sync(); lds_inverse<LOGN, LOGT, NT, RB, STOP, LZ>(...), with the first
+// inverse pass's twiddles loaded before that barrier and each later pass's
+// before its own (where they fit; otherwise exactly those two calls).
+template <int LOGN, int LOGT, int NT, int RB, int STOP, bool LZ>
+__device__ __forceinline__ void sync_inverse_pre(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
+    if constexpr (!inv_pre_fits<LOGN, LOGT, NT, RB, STOP>()) {
+        lds_sync();
+        lds_inverse<LOGN, LOGT, NT, RB, STOP, LZ>(lds, itw, c, tid);
+    } else if constexpr (STOP >= 1) {
+        PassTwOf<LOGN, LOGT, NT, STOP - 1, RB> first;
+        first.load(itw[STOP - 1], tid);
+        lds_sync();
+        inv_pre_from<LOGN, LOGT, NT, STOP - 1, RB, LZ>(lds, itw, c, tid, first);
+    } else {
+        lds_sync();
+    }
+}
+
 // Pass lists: forward runs passes 0..np-1 with shrinking block size; the
 // inverse runs them backwards (post-order recursion).  Passes I >= STOP are
 // left out (the row kernel runs its last pass in registers).
@@ -495,8 +602,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
                 for (int q = 0; q < RL; ++q) ldb[lp(col * RL + q)] = x[q];
             }
         }
-        lds_sync();
-        lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(ldb, L.row_itw[pr], c);
+        sync_inverse_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(ldb, L.row_itw[pr], c, threadIdx.x);
         uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
             out[m] = make_uint4(ldb[lp(4 * m)], ldb[lp(4 * m + 1)], ldb[lp(4 * m + 2)], ldb[lp(4 * m + 3)]);
@@ -562,8 +668,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
             for (int q = 0; q < RL; ++q) lds[lp(col * RL + q)] = x[q];
         }
     }
-    lds_sync();
-    lds_inverse<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_itw[pr], c);
+    sync_inverse_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_itw[pr], c, threadIdx.x);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
         out[m] = make_uint4(lds[lp(4 * m)], lds[lp(4 * m + 1)], lds[lp(4 * m + 2)], lds[lp(4 * m + 3)]);
