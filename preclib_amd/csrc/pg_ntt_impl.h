@@ -168,6 +168,54 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
     }
 }
 
+// Pass lists: forward runs passes 0..np-1 with shrinking block size; the
+// inverse runs them backwards (post-order recursion).  Passes I >= STOP are
+// left out (the row kernel runs its last pass in registers).
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
+__device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, int tid) {
+    constexpr int LR = split_lr(LOGN, I, RB);
+    if constexpr (LR > 0 && I < STOP) {
+        pass_fwd<LR, LOGN, LOGT, LOGB, NT, false, LZ>(lds, tw[I], c, tid);
+        lds_sync();
+        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lds, tw, c, tid);
+    }
+}
+
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
+__device__ __forceinline__ void fwd2_from(uint32_t *lda, uint32_t *ldb, const uint2 *const *tw, const PrimeK &c,
+                                          int tid) {
+    constexpr int LR = split_lr(LOGN, I, RB);
+    if constexpr (LR > 0 && I < STOP) {
+        pass_fwd2<LR, LOGN, LOGT, LOGB, NT, LZ>(lda, ldb, tw[I], c, tid);
+        lds_sync();
+        fwd2_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lda, ldb, tw, c, tid);
+    }
+}
+
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
+__device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
+    constexpr int LR = split_lr(LOGN, I, RB);
+    if constexpr (LR > 0 && I < STOP) {
+        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lds, itw, c, tid);
+        pass_inv<LR, LOGN, LOGT, LOGB, NT, LZ>(lds, itw[I], c, tid);
+        lds_sync();
+    }
+}
+
+// tid: the thread index (callers may pass an opaque copy of threadIdx.x so
+// that index arithmetic is recomputed per call rather than kept live).
+template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS, bool LZ = false>
+__device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c,
+                                            int tid = threadIdx.x) {
+    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lds, tw, c, tid);
+}
+
+template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS, bool LZ = false>
+__device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c,
+                                            int tid = threadIdx.x) {
+    inv_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lds, itw, c, tid);
+}
+
 // Twiddles of one inverse pass for the columns a thread handles, loaded
 // into registers ahead of the pass (before the barrier that precedes it), so
 // the pass starts on its products instead of waiting for the loads.
@@ -218,6 +266,37 @@ __device__ __forceinline__ void pass_inv_pre(uint32_t *lds, const PassTw<LR, LOG
     }
 }
 
+// pass_fwd2 with the pass's twiddles already in registers.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool LZ>
+__device__ __forceinline__ void pass_fwd2_pre(uint32_t *lda, uint32_t *ldb, const PassTw<LR, LOGN, LOGT, LOGB, NT> &tw,
+                                              const PrimeK &c, int tid) {
+    using TW = PassTw<LR, LOGN, LOGT, LOGB, NT>;
+    constexpr int R = TW::R, LOGM = TW::LOGM, M = TW::M, T = TW::T, S = M * T;
+#pragma unroll
+    for (int it = 0; it < TW::IT; ++it) {
+        const int col = tid + it * NT;
+        if (col < TW::NCOLS) {
+            const int cc = col & (T - 1), r = col >> LOGT;
+            const int blk = r >> LOGM, j2 = r & (M - 1);
+            const int base = (((blk << LOGB) + j2) << LOGT) + cc;
+#pragma unroll
+            for (int op = 0; op < 2; ++op) {
+                uint32_t *lds = op ? ldb : lda;
+                uint32_t x[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
+                dft_dif<R, false, LZ>(x, c);
+                if constexpr (M > 1) {
+#pragma unroll
+                    for (int q = 1; q < R; ++q) x[q] = mul_shoup<LZ>(x[q], tw.w[it][q].x, tw.w[it][q].y, c);
+                }
+#pragma unroll
+                for (int q = 0; q < R; ++q) lds[lp(base + bitrev<R>(q) * S)] = x[q];
+            }
+        }
+    }
+}
+
 // Block size (log2) of forward pass I of a 2^LOGN transform.
 template <int LOGN, int I, int RB>
 constexpr int pass_logb() {
@@ -245,6 +324,23 @@ __device__ __forceinline__ void inv_pre_from(uint32_t *lds, const uint2 *const *
     }
 }
 
+// Forward passes I .. STOP - 1 of both tiles (each followed by a barrier),
+// pass I with its twiddles in `cur`; each next pass's twiddles are loaded
+// before the barrier that precedes it.
+template <int LOGN, int LOGT, int NT, int I, int RB, int STOP, bool LZ>
+__device__ __forceinline__ void fwd2_pre_from(uint32_t *lda, uint32_t *ldb, const uint2 *const *tw, const PrimeK &c,
+                                              int tid, const PassTwOf<LOGN, LOGT, NT, I, RB> &cur) {
+    pass_fwd2_pre<split_lr(LOGN, I, RB), LOGN, LOGT, pass_logb<LOGN, I, RB>(), NT, LZ>(lda, ldb, cur, c, tid);
+    if constexpr (I + 1 < STOP) {
+        PassTwOf<LOGN, LOGT, NT, I + 1, RB> nxt;
+        nxt.load(tw[I + 1], tid);
+        lds_sync();
+        fwd2_pre_from<LOGN, LOGT, NT, I + 1, RB, STOP, LZ>(lda, ldb, tw, c, tid, nxt);
+    } else {
+        lds_sync();
+    }
+}
+
 // Whether every inverse pass below STOP keeps at most 15 twiddle pairs per
 // thread (the register budget of the prefetch).
 template <int LOGN, int LOGT, int NT, int RB, int STOP>
@@ -257,12 +353,32 @@ constexpr bool inv_pre_fits() {
     return true;
 }
 
+// lds_sync(); fwd2_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(...), with the
+// first pass's twiddles loaded before that barrier and each later pass's
+// before its own (where they fit; otherwise exactly those two calls).
+// PRE = false (a kernel held to 64 VGPRs) also takes the plain form.
+template <int LOGN, int LOGT, int NT, int RB, int STOP, bool LZ, bool PRE>
+__device__ __forceinline__ void sync_fwd2_pre(uint32_t *lda, uint32_t *ldb, const uint2 *const *tw, const PrimeK &c,
+                                              int tid) {
+    if constexpr (!PRE || !inv_pre_fits<LOGN, LOGT, NT, RB, STOP>()) {
+        lds_sync();
+        fwd2_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lda, ldb, tw, c, tid);
+    } else if constexpr (STOP >= 1) {
+        PassTwOf<LOGN, LOGT, NT, 0, RB> first;
+        first.load(tw[0], tid);
+        lds_sync();
+        fwd2_pre_from<LOGN, LOGT, NT, 0, RB, STOP, LZ>(lda, ldb, tw, c, tid, first);
+    } else {
+        lds_sync();
+    }
+}
+
 // lds_sync(); lds_inverse<LOGN, LOGT, NT, RB, STOP, LZ>(...), with the first
 // inverse pass's twiddles loaded before that barrier and each later pass's
 // before its own (where they fit; otherwise exactly those two calls).
-template <int LOGN, int LOGT, int NT, int RB, int STOP, bool LZ>
+template <int LOGN, int LOGT, int NT, int RB, int STOP, bool LZ, bool PRE>
 __device__ __forceinline__ void sync_inverse_pre(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
-    if constexpr (!inv_pre_fits<LOGN, LOGT, NT, RB, STOP>()) {
+    if constexpr (!PRE || !inv_pre_fits<LOGN, LOGT, NT, RB, STOP>()) {
         lds_sync();
         lds_inverse<LOGN, LOGT, NT, RB, STOP, LZ>(lds, itw, c, tid);
     } else if constexpr (STOP >= 1) {
@@ -273,54 +389,6 @@ __device__ __forceinline__ void sync_inverse_pre(uint32_t *lds, const uint2 *con
     } else {
         lds_sync();
     }
-}
-
-// Pass lists: forward runs passes 0..np-1 with shrinking block size; the
-// inverse runs them backwards (post-order recursion).  Passes I >= STOP are
-// left out (the row kernel runs its last pass in registers).
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
-__device__ __forceinline__ void fwd_from(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, int tid) {
-    constexpr int LR = split_lr(LOGN, I, RB);
-    if constexpr (LR > 0 && I < STOP) {
-        pass_fwd<LR, LOGN, LOGT, LOGB, NT, false, LZ>(lds, tw[I], c, tid);
-        lds_sync();
-        fwd_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lds, tw, c, tid);
-    }
-}
-
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
-__device__ __forceinline__ void fwd2_from(uint32_t *lda, uint32_t *ldb, const uint2 *const *tw, const PrimeK &c,
-                                          int tid) {
-    constexpr int LR = split_lr(LOGN, I, RB);
-    if constexpr (LR > 0 && I < STOP) {
-        pass_fwd2<LR, LOGN, LOGT, LOGB, NT, LZ>(lda, ldb, tw[I], c, tid);
-        lds_sync();
-        fwd2_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lda, ldb, tw, c, tid);
-    }
-}
-
-template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, int STOP, bool LZ = false>
-__device__ __forceinline__ void inv_from(uint32_t *lds, const uint2 *const *itw, const PrimeK &c, int tid) {
-    constexpr int LR = split_lr(LOGN, I, RB);
-    if constexpr (LR > 0 && I < STOP) {
-        inv_from<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, STOP, LZ>(lds, itw, c, tid);
-        pass_inv<LR, LOGN, LOGT, LOGB, NT, LZ>(lds, itw[I], c, tid);
-        lds_sync();
-    }
-}
-
-// tid: the thread index (callers may pass an opaque copy of threadIdx.x so
-// that index arithmetic is recomputed per call rather than kept live).
-template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS, bool LZ = false>
-__device__ __forceinline__ void lds_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c,
-                                            int tid = threadIdx.x) {
-    fwd_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lds, tw, c, tid);
-}
-
-template <int LOGN, int LOGT, int NT, int RB, int STOP = MAX_PASS, bool LZ = false>
-__device__ __forceinline__ void lds_inverse(uint32_t *lds, const uint2 *const *itw, const PrimeK &c,
-                                            int tid = threadIdx.x) {
-    inv_from<LOGN, LOGT, NT, LOGN, 0, RB, STOP, LZ>(lds, itw, c, tid);
 }
 
 // Frequency index held at position rho after the forward passes of a 2^LOGN
@@ -564,6 +632,10 @@ __device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, con
 template <int LOGN, int MODE, int V, bool LZ>
 __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, int pr, int row) {
     constexpr int L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
+    // twiddle prefetch across barriers for rows of 2^13 points and up, whose
+    // LDS (not registers) bounds the occupancy; not in kernels held to 64
+    // VGPRs (measured slower on the shorter rows of batches)
+    constexpr bool PRE = row_waves(LOGN, V) < 8 && L2 >= 13;
     constexpr int RBV = plan_rb(LOGN, V);
     // The last forward pass (block size R_L, no twiddles) runs in registers:
     // thread t holds the outputs of last-pass columns t + it * ROW_THREADS,
@@ -581,8 +653,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
         // registers, and the inverse finishes in B's tile.
         uint32_t *lda = lds, *ldb = lds + N2;
         load_rows<LOGN, MODE, V, 2>(lda, L, D, 0, pr, row);   // A at lda, B at lda + N2 = ldb
-        lds_sync();
-        fwd2_from<L2, 0, ROW_THREADS, L2, 0, RBV, NPS - 1, LZ>(lda, ldb, L.row_tw[pr], c, threadIdx.x);
+        sync_fwd2_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ, PRE>(lda, ldb, L.row_tw[pr], c, threadIdx.x);
 #pragma unroll
         for (int it = 0; it < IL; ++it) {
             const int col = threadIdx.x + it * ROW_THREADS;
@@ -602,7 +673,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
                 for (int q = 0; q < RL; ++q) ldb[lp(col * RL + q)] = x[q];
             }
         }
-        sync_inverse_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(ldb, L.row_itw[pr], c, threadIdx.x);
+        sync_inverse_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ, PRE>(ldb, L.row_itw[pr], c, threadIdx.x);
         uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
             out[m] = make_uint4(ldb[lp(4 * m)], ldb[lp(4 * m + 1)], ldb[lp(4 * m + 2)], ldb[lp(4 * m + 3)]);
@@ -668,7 +739,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
             for (int q = 0; q < RL; ++q) lds[lp(col * RL + q)] = x[q];
         }
     }
-    sync_inverse_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ>(lds, L.row_itw[pr], c, threadIdx.x);
+    sync_inverse_pre<L2, 0, ROW_THREADS, RBV, NPS - 1, LZ, PRE>(lds, L.row_itw[pr], c, threadIdx.x);
     uint4 *out = (uint4 *)(D.prod + ((size_t)pr << LOGN) + ((size_t)row << L2));
     for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS)
         out[m] = make_uint4(lds[lp(4 * m)], lds[lp(4 * m + 1)], lds[lp(4 * m + 2)], lds[lp(4 * m + 3)]);
