@@ -34,8 +34,9 @@ __device__ __forceinline__ const CarryNode &node_of(const CarryLaunch &C, size_t
 }
 
 // Limb j of v (n limbs, zero beyond) shifted right by sh bits (left for sh < 0).
-__device__ __forceinline__ uint64_t shifted_limb(const uint64_t *v, size_t n, int64_t sh, size_t j) {
-    if (!v) return 0;
+__device__ __forceinline__ uint64_t shifted_limb(const uint64_t *vp, size_t n, int64_t sh, size_t j) {
+    if (!vp) return 0;
+    const auto v = gptr(vp);
     if (sh >= 0) {
         const size_t i = j + ((size_t)sh >> 6);
         const int r = (int)(sh & 63);
@@ -50,7 +51,7 @@ __device__ __forceinline__ uint64_t shifted_limb(const uint64_t *v, size_t n, in
     return w;
 }
 
-__device__ __forceinline__ bool lin_cond(const CarryNode &N) { return !N.cond || *N.cond == N.cond_eq; }
+__device__ __forceinline__ bool lin_cond(const CarryNode &N) { return !N.cond || *gptr(N.cond) == N.cond_eq; }
 
 // Limb sum j of a node as (lo, hi).
 __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t &lo, uint64_t &hi) {
@@ -72,12 +73,13 @@ __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t 
         return;
     }
     if (N.kind == NODE_PLAIN) {
+        const auto slo = gptr(N.slo), shi = gptr(N.shi);
         if (N.parts == 1) {
-            lo = N.slo[j];
-            hi = N.shi[j];
+            lo = slo[j];
+            hi = shi[j];
         } else {   // two parts starting at bit 0 and bit 32 of the limb
-            const uint64_t l0 = N.slo[2 * j], h0 = N.shi[2 * j];
-            const uint64_t l1 = N.slo[2 * j + 1], h1 = N.shi[2 * j + 1];
+            const uint64_t l0 = slo[2 * j], h0 = shi[2 * j];
+            const uint64_t l1 = slo[2 * j + 1], h1 = shi[2 * j + 1];
             lo = l0 + (l1 << 32);
             hi = h0 + (l1 >> 32) + (h1 << 32) + (lo < l0);
         }
@@ -90,11 +92,11 @@ __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t 
         if (k < (size_t)d) break;
         const size_t kb = k - d;
         if (kb >= (size_t)N.nblocks) continue;
-        const BlockRef &blk = N.blocks[kb];
+        const auto blk = gptr(N.blocks) + kb;
         const size_t local = j - kb * N.B;
-        if (local < blk.len) {
-            const uint64_t s = lo + blk.slo[local];
-            hi += blk.shi[local] + (s < lo);
+        if (local < blk->len) {
+            const uint64_t s = lo + gptr(blk->slo)[local];
+            hi += gptr(blk->shi)[local] + (s < lo);
             lo = s;
         }
     }
@@ -165,8 +167,7 @@ __device__ __forceinline__ uint32_t node_cin(const CarryLaunch &C, size_t i) {
 template <int IT>
 __device__ __forceinline__ void limbs_t_plain(const CarryNode &N, long base, uint64_t (&t)[IT],
                                               uint32_t (&g)[IT], uint32_t (&p)[IT]) {
-    const uint64_t *__restrict__ slo = N.slo;
-    const uint64_t *__restrict__ shi = N.shi;
+    const auto slo = gptr(N.slo), shi = gptr(N.shi);
     const long m = (long)N.m;
     uint64_t lo[IT], hp[IT];
     if (m) {
@@ -288,10 +289,10 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch 
         const uint32_t x = (uint32_t)(((sum ^ G[it].X ^ G[it].Y) >> lane) & 1);
         const long j = base + 64 * it;
         if ((size_t)j < N.out_len) {
-            N.out[j] = t[it + 1] + x;
-            if (N.out_hi) N.out_hi[j] = 0;
+            gptr_mut(N.out)[j] = t[it + 1] + x;
+            if (N.out_hi) gptr_mut(N.out_hi)[j] = 0;
         }
-        if ((size_t)j == N.m && N.flag) *N.flag = t[it + 1] + x;
+        if ((size_t)j == N.m && N.flag) *gptr_mut(N.flag) = t[it + 1] + x;
         cin = group_out(G[it], cin);
     }
 }
@@ -342,8 +343,8 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C
         const int k = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
         if (lane <= k) {   // k == 64: the whole group propagates
             const uint64_t val = t[0] + 1;
-            if ((size_t)j < N.out_len) N.out[j] = val;
-            if ((size_t)j == N.m && N.flag) *N.flag = val;
+            if ((size_t)j < N.out_len) gptr_mut(N.out)[j] = val;
+            if ((size_t)j == N.m && N.flag) *gptr_mut(N.flag) = val;
         }
         if (stop) return;
     }

@@ -7,6 +7,18 @@
 
 namespace pg {
 
+// Global-memory view of a device pointer read from a descriptor: accesses
+// through it are global (not flat) instructions.  A flat access also counts
+// against the LDS counter, so LDS reads after it wait for it.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr_mut(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 constexpr int NTT_THREADS = 1024;           // workgroup size of the column kernels
 #ifndef PG_COLINV_THREADS
 #define PG_COLINV_THREADS 1024

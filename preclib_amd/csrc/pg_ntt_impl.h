@@ -63,16 +63,6 @@ __device__ __forceinline__ int fresh_tid() {
     return t;
 }
 
-// Global-memory view of a device pointer read from a descriptor: loads
-// through it are global_load (not flat) instructions.
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const T *p) {
-    return (const __attribute__((address_space(1))) T *)p;
-}
-template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T *gptr_mut(T *p) {
-    return (__attribute__((address_space(1))) T *)p;
-}
 // 16-byte vector of the compiler's own (loads through an address-space
 // pointer need a builtin vector type; HIP's uint4 is a class)
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
