@@ -31,6 +31,25 @@ HBM_PEAK = 8.0e12          # MI355X published HBM3E bandwidth, bytes/s
 # (4 SIMDs of 16 lanes, 4 cycles each), 2.4 GHz peak engine clock
 VALU_PEAK = 256 * 2.4e9    # wave-instructions/s (= 39.3e12 32-bit lane ops/s)
 NPRIMES = 2
+RAMP_S = 0.06              # untimed load before the warm-up steps (clock ramp)
+
+
+def host_cpu():
+    """Model name of the host CPU and the number of cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except Exception:
+        ncpu = os.cpu_count()
+    return {"host_cpu": model, "host_nproc": ncpu, "host_cpu_count": os.cpu_count()}
 
 
 def model_bytes(an, bn, n):
@@ -93,9 +112,11 @@ def cpu_baseline(a, b, budget_s=25.0):
         if time.perf_counter() - t_start + times[-1] > budget_s or len(times) >= 3:
             break
     med = sorted(times)[len(times) // 2]
-    return {"value": (len(a) + len(b)) / med, "unit": "product limbs/s", "cores": 1, "kind": kind,
-            "sample": "%d full %dx%d-limb products (median %.2f s), pinned to one host core"
-                      % (len(times), len(a), len(b), med)}
+    out = {"value": (len(a) + len(b)) / med, "unit": "product limbs/s", "cores": 1, "kind": kind,
+           "sample": "%d full %dx%d-limb products (median %.2f s), pinned to one host core"
+                     % (len(times), len(a), len(b), med)}
+    out.update(host_cpu())
+    return out
 
 
 def ref_leaf_bytes(an, bn, sqr=False):
@@ -400,6 +421,7 @@ def main():
     N = 1 << args.logn
     a, b = workloads.pair(N, N, args.dist)
     n = transform_len(a, b)
+    nw = max(1, n // 2)   # base-2^32 digits: half as many points
     da, db = pgpu.DevBuf.of(a), pgpu.DevBuf.of(b)
     r = pgpu.DevBuf(2 * N)
     # correctness of the measured configuration (outside the timed region,
@@ -416,6 +438,13 @@ def main():
         print("bench.py: product differs from the golden digest", file=sys.stderr)
         sys.exit(1)
 
+    # clock ramp: the engine clock rises over the first ~30 ms of load, so the
+    # same products keep the GPU busy for RAMP_S before the W warm-up steps
+    t_ramp = time.perf_counter()
+    while time.perf_counter() - t_ramp < RAMP_S:
+        for _ in range(8):
+            pgpu.mul_dev(r, da, N, db, N, want_len=False)
+        pgpu.sync()
     for _ in range(args.warmup):
         pgpu.mul_dev(r, da, N, db, N, want_len=False)
     # timed region: K products back to back, no per-stage events between the
@@ -482,9 +511,13 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 modular (NTT), u64 limbs",
         "data": "synthetic (splitmix64 %s limbs, top limb nonzero)" % args.dist,
-        "config": {"workload": "single product %dx%d limbs, NTT n=2^%d, P=%d primes" %
-                   (N, N, n.bit_length() - 1, NPRIMES), "an": N, "bn": N, "dist": args.dist,
-                   "bit_exact_vs_golden": check},
+        "config": {"workload": "single product %dx%d limbs" % (N, N), "an": N, "bn": N, "dist": args.dist,
+                   "executed": {"digits": "base 2^32", "primes": 3, "transform_len": "2^%d" % (nw.bit_length() - 1),
+                                "plan_variant": pgpu.ntt_plans()[nw.bit_length() - 1]},
+                   "byte_model": {"digits": "base 2^16", "primes": NPRIMES,
+                                  "transform_len": "2^%d" % (n.bit_length() - 1),
+                                  "bytes": 16.0 * 2 * N + 48.0 * NPRIMES * n},
+                   "clock_ramp_s": RAMP_S, "bit_exact_vs_golden": check},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3},

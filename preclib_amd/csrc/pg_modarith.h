@@ -39,6 +39,9 @@ __device__ __forceinline__ uint32_t mod_sub(uint32_t a, uint32_t b, uint32_t p) 
 // differences are reduced against 2p, and products by a fixed twiddle skip
 // Shoup's last correction.  Every operand stays below 4p < 2^32.
 constexpr uint32_t LAZY_PRIME_LIMIT = 1u << 30;
+#ifndef PG_DIF_RAW
+#define PG_DIF_RAW 1
+#endif
 
 // x * w mod c.p with wq = floor(w * 2^32 / p); x < 2^32, w < p.  With
 // q = floor(x wq / 2^32), x w - q p lies in [0, 2p); one min() brings it to
@@ -74,7 +77,9 @@ __device__ __forceinline__ constexpr int bitrev(int q) {
 // x[q] = X[bitrev(q)] out.  Twiddles omega_R^k = omega_32^(k * 32 / R).
 // HALFZ: x[R/2 .. R) are zero on entry, so the first stage is only the
 // twiddle products of the lower half.  LZ: residues in [0, 2p) in and out.
-template <int R, bool HALFZ = false, bool LZ = false>
+// RAW: outputs x[1 .. R) feed Shoup products (which take any x < 2^32), so
+// the last stage leaves them unreduced, below 2 * (reduction bound) <= 2^32.
+template <int R, bool HALFZ = false, bool LZ = false, bool RAW = false>
 __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
     const uint32_t p = c.p, pr = LZ ? 2 * c.p : c.p;   // reduction bound
     if constexpr (HALFZ) {
@@ -89,6 +94,11 @@ __device__ __forceinline__ void dft_dif(uint32_t (&x)[R], const PrimeK &c) {
 #pragma unroll
             for (int j = 0; j < h; ++j) {
                 uint32_t u = x[b + j], v = x[b + j + h];
+                if (PG_DIF_RAW && RAW && h == 1) {
+                    x[b] = b == 0 ? mod_add(u, v, pr) : u + v;
+                    x[b + 1] = u - v + pr;
+                    continue;
+                }
                 x[b + j] = mod_add(u, v, pr);
                 if (j == 0) {
                     x[b + j + h] = mod_sub(u, v, pr);

@@ -63,6 +63,27 @@ __device__ __forceinline__ int fresh_tid() {
     return t;
 }
 
+// The workgroup's leaf descriptor, copied into registers once: read through
+// a reference, its fields are reloaded by flat loads at every use, and each
+// flat load waits for every global store issued before it.
+// The fields are uniform: readfirstlane keeps them in scalar registers.
+__device__ __forceinline__ NttLeaf leaf_of(const NttLaunch &L) {
+    union {
+        NttLeaf d;
+        int w[sizeof(NttLeaf) / 4];
+    } u;
+    if (L.leaves) {
+        const auto src = gptr((const int *)(L.leaves + blockIdx.z));
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(NttLeaf) / 4); ++i) u.w[i] = src[i];
+    } else {
+        u.d = L.one;
+    }
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(NttLeaf) / 4); ++i) u.w[i] = __builtin_amdgcn_readfirstlane(u.w[i]);
+    return u.d;
+}
+
 // 16-byte vector of the compiler's own (loads through an address-space
 // pointer need a builtin vector type; HIP's uint4 is a class)
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -85,7 +106,7 @@ __device__ __forceinline__ void pass_fwd(uint32_t *lds, const uint2 *__restrict_
         uint32_t x[R];
 #pragma unroll
         for (int q = 0; q < (HALFZ ? R / 2 : R); ++q) x[q] = lds[lp(base + q * S)];
-        dft_dif<R, HALFZ, LZ>(x, c);
+        dft_dif<R, HALFZ, LZ, (M > 1)>(x, c);
         if constexpr (M > 1) {
             const uint2 *t = tw + j2;
 #pragma unroll
@@ -124,7 +145,7 @@ __device__ __forceinline__ void pass_fwd2(uint32_t *lda, uint32_t *ldb, const ui
             uint32_t x[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
-            dft_dif<R, false, LZ>(x, c);
+            dft_dif<R, false, LZ, (M > 1)>(x, c);
             if constexpr (M > 1) {
 #pragma unroll
                 for (int q = 1; q < R; ++q) x[q] = mul_shoup<LZ>(x[q], w[q].x, w[q].y, c);
@@ -279,7 +300,7 @@ __device__ __forceinline__ void pass_fwd2_pre(uint32_t *lda, uint32_t *ldb, cons
                 uint32_t x[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) x[q] = lds[lp(base + q * S)];
-                dft_dif<R, false, LZ>(x, c);
+                dft_dif<R, false, LZ, (M > 1)>(x, c);
                 if constexpr (M > 1) {
 #pragma unroll
                     for (int q = 1; q < R; ++q) x[q] = mul_shoup<LZ>(x[q], tw.w[it][q].x, tw.w[it][q].y, c);
@@ -494,7 +515,7 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
     constexpr int NP = mode_primes(MODE);
     using TW = ColTw<L1, LT, RBV, NTT_THREADS>;
     extern __shared__ uint32_t lds[];
-    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const NttLeaf D = leaf_of(L);
     int tile, pr, op;
     switch (L.order & 3) {
     case COLF_OP_PRIME:
@@ -747,7 +768,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
 template <int LOGN, int MODE, int V>
 __global__ void __launch_bounds__(row_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(row_waves(LOGN, V))))
 k_row(const NttLaunch L) {
-    const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
+    const NttLeaf D = leaf_of(L);
     const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
     if (pr == LAZY_PRIME) row_body<LOGN, MODE, V, true>(L, D, pr, row);
     else row_body<LOGN, MODE, V, false>(L, D, pr, row);
@@ -985,6 +1006,9 @@ __global__ void __launch_bounds__(COLINV_THREADS)
 __attribute__((amdgpu_waves_per_eu(MODE == MODE_WIDE ? col_inv_waves<LOGN, V>() : 1)))
 k_col_inv(const NttLaunch L) {
     extern __shared__ uint32_t lds[];
+    // (read through the reference here: leaf_of's scalar registers push this
+    // kernel to 106 SGPRs and its pass twiddle loads are then issued one at
+    // a time, 208 -> 227 us at 2^24 points)
     const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const uint32_t c0 = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) << plan_lt(LOGN, V);
     if constexpr (MODE == MODE_WIDE) col_inv_wide<LOGN, V>(lds, L, D, c0);

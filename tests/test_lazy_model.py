@@ -62,7 +62,10 @@ def omega32(p, g, inverse=False):
     return pow(w, p - 2, p) if inverse else w
 
 
-def dft_dif(x, A, w32):
+def dft_dif(x, A, w32, raw=False):
+    """raw: the last stage leaves outputs 1 .. r-1 unreduced (u + v and
+    u - v + pr), for a pass whose outputs all but the first go straight into
+    a twiddle product."""
     r = len(x)
     x = list(x)
     h = r // 2
@@ -70,6 +73,14 @@ def dft_dif(x, A, w32):
         for b in range(0, r, 2 * h):
             for j in range(h):
                 u, v = x[b + j], x[b + j + h]
+                if raw and h == 1:
+                    if b == 0:
+                        x[b] = A.add(u, v)
+                    else:
+                        x[b] = u + v
+                        assert x[b] <= M32, "unreduced sum wraps"
+                    x[b + 1] = A.diff_in(u, v)
+                    continue
                 x[b + j] = A.add(u, v)
                 if j == 0:
                     x[b + j + h] = A.sub(u, v)
@@ -132,6 +143,32 @@ def test_register_dfts(p, lazy, r):
         # and from its definition
         yb = [y[bitrev(q, r)] % p for q in range(r)]
         assert [v % p for v in z] == naive_dft(yb, p, iwr)
+
+
+@pytest.mark.parametrize("p,lazy", [(469762049, True), (469762049, False),
+                                    (1811939329, False), (2013265921, False)])
+@pytest.mark.parametrize("r", [2, 4, 8, 16])
+def test_raw_outputs_into_twiddles(p, lazy, r):
+    """A forward pass with inter-pass twiddles: the register DFT's last stage
+    leaves outputs 1 .. r-1 unreduced and Shoup's product takes them (any
+    x < 2^32); the twiddled outputs equal those of the reduced form."""
+    g = PRIMES[p]
+    w = omega32(p, g)
+    w32 = [pow(w, i, p) for i in range(16)]
+    A = Arith(p, lazy)
+    rng = random.Random(3 * p + r)
+    tw = [rng.randrange(p) for _ in range(r)]
+    for x in cases(p, A.top, r, rng):
+        y0 = dft_dif(x, A, w32)
+        y1 = dft_dif(x, A, w32, raw=True)
+        assert y1[0] == y0[0] and 0 <= y1[0] < A.top
+        assert all(y1[q] < 2 * A.pr for q in range(1, r))
+        assert [v % p for v in y1] == [v % p for v in y0]
+        t0 = [y0[0]] + [A.shoup(y0[q], tw[q]) for q in range(1, r)]
+        t1 = [y1[0]] + [A.shoup(y1[q], tw[q]) for q in range(1, r)]
+        # lazy residues may differ by p as representatives; canonical ones agree exactly
+        assert [v % p for v in t0] == [v % p for v in t1] and all(0 <= v < A.top for v in t1)
+        assert lazy or t0 == t1
 
 
 def test_lazy_needs_the_small_prime():
