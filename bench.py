@@ -288,31 +288,28 @@ def bench_pi(args, pgpu):
     print(json.dumps(line), flush=True)
 
 
-def serve_requests(count, seed, workloads):
-    """C5 requests: 80% multiply (both sizes log-uniform in [256, 2^21]),
-    20% division (divisor log-uniform in [32768, 2^20], dividend above twice
-    the divisor up to 2^21); uniform limbs."""
-    rng = np.random.default_rng(seed)
-    out = []
-    lo, hi = np.log(256), np.log(1 << 21)
-    for k in range(count):
-        if rng.uniform() < 0.8:
-            an, bn = (int(np.exp(rng.uniform(lo, hi))) for _ in range(2))
-            op = 0
-        else:
-            bn = int(np.exp(rng.uniform(np.log(32768), np.log(1 << 20))))
-            an = int(np.exp(rng.uniform(np.log(2 * bn + 1), hi)))
-            op = 1
-        a, b = workloads.pair(an, bn, "uniform", 5000 + k)
-        out.append((op, a, b))
-    return out
-
-
 def bench_serve(args, pgpu, workloads):
-    """Poisson request stream through the continuous-batching server."""
+    """Poisson request stream through the continuous-batching server: the
+    serving stream (workloads.serve_stream: 80% multiplies, sizes log-uniform
+    in [256, 2^21]; 20% divisions, divisor >= 32768 limbs, dividend above
+    twice the divisor).  Every result of every run is compared with the same
+    request run alone through pg_mul / pg_div."""
     import threading
-    reqs = serve_requests(args.requests, 0x5E12E, workloads)
+    reqs = workloads.serve_stream(args.requests)
     limbs = [len(a) + len(b) for _, a, b in reqs]
+    alone = [pgpu.mul(a, b) if op == 0 else pgpu.divmod_(a, b) for op, a, b in reqs]
+
+    def same(x, y):
+        if isinstance(x, tuple):
+            return all(np.array_equal(u, v) for u, v in zip(x, y))
+        return np.array_equal(x, y)
+
+    def verify(hs, what):
+        for i, h in enumerate(hs):
+            if not same(h.result(), alone[i]):
+                print("bench.py: %s: request %d differs from the same request alone" % (what, i), file=sys.stderr)
+                sys.exit(1)
+
     # saturated throughput: every request offered at once
     srv = pgpu.Server()
     warm = [srv.submit(op, a, b) for op, a, b in reqs[:8]]
@@ -325,14 +322,8 @@ def bench_serve(args, pgpu, workloads):
     sat_rate = len(reqs) / sat_s
     st = srv.stats()
     srv.stop()
-    # alone vs mix: the first few requests alone through the plain entry points
-    for (op, a, b), h in list(zip(reqs, hs))[:6]:
-        r = h.result()
-        ref = pgpu.mul(a, b) if op == 0 else pgpu.divmod_(a, b)
-        same = np.array_equal(r, ref) if op == 0 else all(np.array_equal(x, y) for x, y in zip(r, ref))
-        if not same:
-            print("bench.py: server result differs from the same request alone", file=sys.stderr)
-            sys.exit(1)
+    verify(hs, "saturated")
+    del hs
     levels = []
     for frac in (0.5, 0.8, 1.1):
         rate = frac * sat_rate
@@ -356,12 +347,18 @@ def bench_serve(args, pgpu, workloads):
         srv.drain()
         wall = time.perf_counter() - t0
         lat = np.array([h.latency for h in hs]) * 1e3
-        for h in hs:
-            h.result()
+        verify(hs, "%.1fx load" % frac)
+        mul_lat = np.array([lat[i] for i, r in enumerate(reqs) if r[0] == 0])
+        div_lat = np.array([lat[i] for i, r in enumerate(reqs) if r[0] == 1])
         levels.append({"offered_frac_of_saturation": frac, "offered_req_s": rate, "achieved_req_s": len(reqs) / wall,
                        "product_limbs_s": sum(limbs) / wall, "p50_ms": float(np.percentile(lat, 50)),
-                       "p99_ms": float(np.percentile(lat, 99)), "max_active": srv.stats()["max_active"]})
+                       "p99_ms": float(np.percentile(lat, 99)),
+                       "mul_p50_ms": float(np.percentile(mul_lat, 50)), "mul_p99_ms": float(np.percentile(mul_lat, 99)),
+                       "div_p50_ms": float(np.percentile(div_lat, 50)), "div_p99_ms": float(np.percentile(div_lat, 99)),
+                       "max_active": srv.stats()["max_active"], "rounds": srv.stats()["rounds"],
+                       "all_equal_alone": True})
         srv.stop()
+        del hs
     line = {
         "metric": "served product limbs/s (80% mul / 20% Newton division, Poisson arrivals)",
         "value": sum(limbs) / sat_s, "unit": "product limbs/s", "n_gpus": 1, "steps": len(reqs), "warmup": 8,
@@ -369,7 +366,7 @@ def bench_serve(args, pgpu, workloads):
         "dtype": "u32 modular (NTT) / f64 (FFT), u64 limbs", "data": "synthetic (splitmix64 uniform limbs)",
         "config": {"workload": "serving stream, sizes log-uniform [256, 2^21], divisor >= 32768",
                    "requests": len(reqs), "divisions": sum(1 for r in reqs if r[0] == 1),
-                   "host_to_host": True},
+                   "host_to_host": True, "checked": "every result of every run equals the request alone"},
         "saturated": {"req_s": sat_rate, "rounds": st["rounds"], "products": st["products"],
                       "max_active": st["max_active"]},
         "levels": levels,

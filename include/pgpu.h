@@ -217,6 +217,13 @@ int pg_ntt_plan_set(const int *variants, int n);
 /* Waits for all queued device work. */
 int pg_sync(void);
 
+/* Diagnostic, no counterpart in the reference: bytes of page-locked host
+ * memory the library holds for descriptor uploads still in flight
+ * (held_bytes, at most 256 MiB plus one image between two waits) and kept
+ * for reuse (spare_bytes, at most 256 MiB).  Every entry point that waits for
+ * the device recycles the held images. */
+int pg_host_staging(size_t *held_bytes, size_t *spare_bytes);
+
 /* Per-stage kernel timing on the library's stream (HIP events around each
  * launch).  pg_timing_read fills up to `cap` entries and returns the number of
  * stages; names are stable identifiers such as "ntt_col_fwd". */

@@ -136,3 +136,28 @@ void or_fill(uint64_t *a, size_t n, uint64_t seed, int dist) {
         a[n - 1] = v ? v : 1;
     }
 }
+
+/* t mod (2^k - c) for t < 2^80: fold the bits above k back in as hi * c. */
+static uint64_t fold(u128 t, int k, uint64_t c) {
+    const u128 p = ((u128)1 << k) - c, mask = ((u128)1 << k) - 1;
+    while (t >> k) t = (t >> k) * c + (t & mask);
+    while (t >= p) t -= p;
+    return (uint64_t)t;
+}
+
+uint64_t or_mod_p(const uint64_t *a, size_t n, int k, uint64_t c) {
+    const uint64_t m = c << (64 - k);   /* 2^64 mod p */
+    uint64_t r = 0;
+    for (size_t i = n; i-- > 0;) r = fold((u128)r * m + a[i], k, c);
+    return r;
+}
+
+void or_mod_p3(const uint64_t *a, size_t n, uint64_t out[3]) {
+    uint64_t r0 = 0, r1 = 0, r2 = 0;
+    for (size_t i = n; i-- > 0;) {
+        r0 = fold((u128)r0 * 8 + a[i], 61, 1);     /* 2^64 = 2^3 mod 2^61 - 1 */
+        r1 = fold((u128)r1 * 50 + a[i], 63, 25);   /* 2^64 = 2 * 25 mod 2^63 - 25 */
+        r2 = fold((u128)r2 * 59 + a[i], 64, 59);   /* 2^64 = 59 mod 2^64 - 59 */
+    }
+    out[0] = r0; out[1] = r1; out[2] = r2;
+}

@@ -81,6 +81,17 @@ size_t or_recip(uint64_t *x, size_t xcap, const uint64_t *b, size_t bn, size_t b
 /* floor(sqrt(a)) (sqrt.cpp:71-86). */
 size_t or_sqrt(uint64_t *x, size_t xcap, const uint64_t *a, size_t an);
 
+/* ---- checks ------------------------------------------------------------ */
+/* (sum a[i] 2^(64 i)) mod p for a pseudo-Mersenne modulus p = 2^k - c,
+ * 32 <= k <= 64, 1 <= c < 2^16 (Horner from the top limb; 2^64 = c 2^(64-k)
+ * mod p, so every step is a small product and a fold, no division).
+ * No reference counterpart: the result checks of the serving tests
+ * (C == A B and A == Q B + R modulo several primes). */
+uint64_t or_mod_p(const uint64_t *a, size_t n, int k, uint64_t c);
+/* The same for the three moduli 2^61 - 1, 2^63 - 25 and 2^64 - 59 in one
+ * pass (independent chains); out[3]. */
+void or_mod_p3(const uint64_t *a, size_t n, uint64_t out[3]);
+
 /* ---- synthetic operands ------------------------------------------------ */
 /* splitmix64 stream (Vigna's published generator). dist: 0 uniform, 1 all
  * ones, 2 sparse (1% nonzero).  The top limb is always nonzero. */

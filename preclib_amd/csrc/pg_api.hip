@@ -74,7 +74,7 @@ int pg_buf_upload(pg_buf *h, const uint64_t *src, size_t limbs) try {
     NEED_INIT;
     if (!h || (!src && limbs) || limbs > h->cap) return set_error(PG_EINVAL, "pg_buf_upload: bad argument");
     if (hipMemcpyAsync(h->d, src, limbs * 8, hipMemcpyHostToDevice, ctx().stream) != hipSuccess ||
-        hipStreamSynchronize(ctx().stream) != hipSuccess)
+        ctx().wait() != hipSuccess)
         return set_error(PG_EDEVICE, "pg_buf_upload: copy failed");
     return PG_OK;
 } catch (...) {
@@ -86,7 +86,7 @@ int pg_buf_download(const pg_buf *h, uint64_t *dst, size_t limbs) try {
     NEED_INIT;
     if (!h || (!dst && limbs) || limbs > h->cap) return set_error(PG_EINVAL, "pg_buf_download: bad argument");
     if (hipMemcpyAsync(dst, h->d, limbs * 8, hipMemcpyDeviceToHost, ctx().stream) != hipSuccess ||
-        hipStreamSynchronize(ctx().stream) != hipSuccess)
+        ctx().wait() != hipSuccess)
         return set_error(PG_EDEVICE, "pg_buf_download: copy failed");
     return PG_OK;
 } catch (...) {
@@ -134,7 +134,7 @@ int pg_mul(uint64_t *r, size_t r_cap, size_t *r_len, const uint64_t *a, size_t a
     if (!st) st = mul_device(dr, da, an, a[an - 1], db, bn, b[bn - 1], path, fft_max_round);
     if (!st && hipMemcpyAsync(r, dr, (an + bn) * 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess)
         st = set_error(PG_EDEVICE, "pg_mul: download failed");
-    if (hipStreamSynchronize(C.stream) != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul: device error");
+    if (C.wait() != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul: device error");
     release();
     if (st) return st;
     size_t len = an + bn;
@@ -162,7 +162,7 @@ int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an, const pg_bu
     if (r_len) {
         uint64_t top = 0;
         if (hipMemcpyAsync(&top, r->d + an + bn - 1, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
-            hipStreamSynchronize(C.stream) != hipSuccess)
+            C.wait() != hipSuccess)
             return set_error(PG_EDEVICE, "pg_mul_dev: device error");
         *r_len = top ? an + bn : an + bn - 1;
     }
@@ -216,7 +216,7 @@ int pg_batch_errors(pg_batch *h, double *err, size_t n) try {
 int pg_batch_free(pg_batch *h) try {
     LOCK;
     if (h) {
-        if (ctx().stream) hipStreamSynchronize(ctx().stream);
+        if (ctx().stream) ctx().wait();
         delete h;
     }
     return PG_OK;
@@ -264,7 +264,7 @@ int pg_mul_batch(pg_job *jobs, size_t n) try {
             if (hipMemcpyAsync(jobs[i].r, dj[i].r, (jobs[i].an + jobs[i].bn) * 8, hipMemcpyDeviceToHost,
                                C.stream) != hipSuccess)
                 st = set_error(PG_EDEVICE, "pg_mul_batch: download failed");
-        if (hipStreamSynchronize(C.stream) != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul_batch: device error");
+        if (C.wait() != hipSuccess && !st) st = set_error(PG_EDEVICE, "pg_mul_batch: device error");
     }
     C.pool.put(dev);
     for (size_t i = 0; i < n; ++i) {
@@ -481,7 +481,7 @@ int pg_timing_enable(int on) try {
 
 int pg_timing_reset(void) try {
     LOCK;
-    if (ctx().stream) hipStreamSynchronize(ctx().stream);
+    if (ctx().stream) ctx().wait();
     ctx().collect_timing();
     ctx().stats.clear();
     return PG_OK;
@@ -489,9 +489,20 @@ int pg_timing_reset(void) try {
     return pg::exception_status();
 }
 
+int pg_host_staging(size_t *held_bytes, size_t *spare_bytes) try {
+    LOCK;
+    size_t sp = 0;
+    for (const auto &h : ctx().spare) sp += h.capacity();
+    if (held_bytes) *held_bytes = ctx().held_bytes;
+    if (spare_bytes) *spare_bytes = sp;
+    return PG_OK;
+} catch (...) {
+    return pg::exception_status();
+}
+
 int pg_timing_read(pg_stage_time *out, int cap) try {
     LOCK;
-    if (ctx().stream) hipStreamSynchronize(ctx().stream);
+    if (ctx().stream) ctx().wait();
     ctx().collect_timing();
     int i = 0;
     for (auto &kv : ctx().stats) {

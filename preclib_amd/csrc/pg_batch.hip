@@ -448,7 +448,7 @@ int Batch::read_errors(std::vector<double> &err) {
     for (size_t i = 0; i < fft_err.size(); ++i)
         if (hipMemcpyAsync(&bits[i], fft_err[i].second, 8, hipMemcpyDeviceToHost, C.stream) != hipSuccess)
             return set_error(PG_EDEVICE, "pg_batch: reading rounding distances failed");
-    if (hipStreamSynchronize(C.stream) != hipSuccess) return set_error(PG_EDEVICE, "pg_batch: device error");
+    if (C.wait() != hipSuccess) return set_error(PG_EDEVICE, "pg_batch: device error");
     for (size_t i = 0; i < fft_err.size(); ++i) {
         double e;
         std::memcpy(&e, &bits[i], 8);

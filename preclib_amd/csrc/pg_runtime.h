@@ -152,11 +152,18 @@ public:
     // Host descriptor images handed to hipMemcpyAsync stay alive until the
     // next sync() of the stream; their buffers are then kept for reuse, so a
     // later image of similar size needs neither fresh pages nor a zero fill.
+    // Every entry point that waits for the stream recycles them (sync() or
+    // wait()); a run of asynchronous calls that holds more than
+    // HELD_MAX bytes waits for the stream in hold().
+    static constexpr size_t HELD_MAX = (size_t)256 << 20;
     std::vector<HostImg> held, spare;
+    size_t held_bytes = 0;
     HostImg take_img(size_t bytes);
-    void hold(HostImg &&img) { held.push_back(std::move(img)); }
+    void hold(HostImg &&img);
     void recycle_held();
     int sync();
+    // hipStreamSynchronize, then recycle_held (without touching the error text)
+    hipError_t wait();
 
     // timing
     bool timing = false;

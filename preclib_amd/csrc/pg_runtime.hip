@@ -184,6 +184,7 @@ void Context::shutdown() {
     if (words) hipFree(words);
     words = nullptr;
     held.clear();
+    held_bytes = 0;
     spare.clear();
     pool.release_all();
     hipStreamDestroy(stream);
@@ -222,14 +223,29 @@ int Context::sync() {
     return PG_OK;
 }
 
+hipError_t Context::wait() {
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) recycle_held();
+    return e;
+}
+
+void Context::hold(HostImg &&img) {
+    held_bytes += img.capacity();
+    held.push_back(std::move(img));
+    // the uploads from every held image are already enqueued: once the stream
+    // has passed them the buffers can be reused
+    if (held_bytes > HELD_MAX && hipStreamSynchronize(stream) == hipSuccess) recycle_held();
+}
+
 // The stream has passed every upload from the held images: keep their buffers
-// (up to 256 MiB, largest first) for take_img.
+// (up to HELD_MAX bytes, largest first) for take_img.
 void Context::recycle_held() {
     for (auto &h : held) spare.push_back(std::move(h));
     held.clear();
+    held_bytes = 0;
     std::sort(spare.begin(), spare.end(), [](const HostImg &a, const HostImg &b) { return a.capacity() > b.capacity(); });
     size_t kept = 0, n = 0;
-    while (n < spare.size() && kept + spare[n].capacity() <= ((size_t)256 << 20)) kept += spare[n++].capacity();
+    while (n < spare.size() && kept + spare[n].capacity() <= HELD_MAX) kept += spare[n++].capacity();
     spare.resize(n);
 }
 

@@ -352,8 +352,12 @@ void Server::round() {
         const int sst = C.sync();
         if (!st) st = sst;
     }
-    stats.rounds++;
-    stats.products += prod.size();
+    {
+        // pg_server_read_stats copies the counters under mu
+        std::lock_guard<std::mutex> g(mu);
+        stats.rounds++;
+        stats.products += prod.size();
+    }
     for (auto &q : active) {
         if (st) fail(*q, st);
         else if (q->stage != S_DONE) advance(*q);

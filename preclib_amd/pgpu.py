@@ -25,7 +25,7 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev", "pg_sqrt", "pg_to_decimal",
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
            "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
-           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set"]
+           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_host_staging"]
 NTT_VARIANTS = 16
 OP_MUL, OP_DIV = 0, 1
 
@@ -211,6 +211,14 @@ def mul_dev(r, a, an, b, bn, path=PATH_AUTO, want_len=True):
 
 def sync():
     _check(_lib.pg_sync())
+
+
+def host_staging():
+    """(held, spare) bytes of page-locked descriptor images (pg_host_staging)."""
+    L = load()
+    held, spare = SZ(0), SZ(0)
+    _check(L.pg_host_staging(ctypes.byref(held), ctypes.byref(spare)))
+    return held.value, spare.value
 
 
 def timing(on=True):

@@ -77,6 +77,35 @@ def batch_shapes(count=4096, lo=193, hi=65536, unbalanced=0.30, seed=0x5EED5EED)
     return [(int(x), int(y)) for x, y in zip(an, bn)]
 
 
+SERVE_SEED = 0x5E12E
+
+
+def serve_shapes(count, seed=SERVE_SEED):
+    """The serving stream's requests as (op, an, bn, k): 80% multiplies with
+    both sizes log-uniform in [256, 2^21] limbs, 20% divisions with the
+    divisor log-uniform in [32768, 2^20] and the dividend log-uniform in
+    (2 * divisor, 2^21]; request k's operands are pair(an, bn, "uniform",
+    5000 + k)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = np.log(256), np.log(1 << 21)
+    for k in range(count):
+        if rng.uniform() < 0.8:
+            an, bn = (int(np.exp(rng.uniform(lo, hi))) for _ in range(2))
+            op = 0
+        else:
+            bn = int(np.exp(rng.uniform(np.log(32768), np.log(1 << 20))))
+            an = int(np.exp(rng.uniform(np.log(2 * bn + 1), hi)))
+            op = 1
+        out.append((op, an, bn, 5000 + k))
+    return out
+
+
+def serve_stream(count, seed=SERVE_SEED):
+    """serve_shapes with the operands: a list of (op, a, b)."""
+    return [(op,) + pair(an, bn, "uniform", k) for op, an, bn, k in serve_shapes(count, seed)]
+
+
 def reference_pattern(n, seed):
     """bench_gmp.cpp / bench_ntt.cpp pattern(): 64-bit LCG, limb = x | 1."""
     out = np.empty(n, np.uint64)

@@ -73,6 +73,9 @@ class Oracle:
         L.or_recip.argtypes = [P64, SZ, P64, SZ, SZ]
         L.or_sqrt.restype = SZ
         L.or_sqrt.argtypes = [P64, SZ, P64, SZ]
+        L.or_mod_p.restype = ctypes.c_uint64
+        L.or_mod_p.argtypes = [P64, SZ, ctypes.c_int, ctypes.c_uint64]
+        L.or_mod_p3.argtypes = [P64, SZ, P64]
 
     def _out(self, an, bn):
         return np.zeros(max(an + bn, 1), np.uint64)
@@ -170,6 +173,20 @@ class Oracle:
         n = self.lib.or_sqrt(_p64(x), len(x), _p64(a), len(a))
         assert n != 2**64 - 1
         return x[:n].copy()
+
+    def mod_p(self, a, k, c):
+        """a mod (2^k - c) of a little-endian limb array."""
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        return int(self.lib.or_mod_p(_p64(a), len(a), k, c))
+
+    MODS3 = ((1 << 61) - 1, (1 << 63) - 25, (1 << 64) - 59)
+
+    def mod_p3(self, a):
+        """a mod (2^61 - 1, 2^63 - 25, 2^64 - 59) in one pass."""
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        out = np.zeros(3, np.uint64)
+        self.lib.or_mod_p3(_p64(a), len(a), _p64(out))
+        return tuple(int(v) for v in out)
 
     def fill(self, n, seed, dist):
         a = np.zeros(n, np.uint64)

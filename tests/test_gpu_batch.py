@@ -99,3 +99,21 @@ def test_device_batch_graph_replays(pg, oracle):
         x, y = W.pair(3000, 3000, "uniform", k=rep)   # eager work between replays
         assert np.array_equal(pg.mul(x, y), oracle.mul_into(x, y))
     batch.free()
+
+
+def test_staging_images_recycled(pg):
+    """Small and unbalanced products upload page-locked descriptor images;
+    every synchronous call recycles them, so a long run of pg_mul calls
+    keeps the staging memory bounded (held 0 after each call, spare at most
+    256 MiB) and the products stay correct."""
+    cap = 256 << 20
+    for k in range(300):
+        an, bn = [(40, 30), (30, 20000), (200, 900), (5, 7000)][k % 4]
+        a, b = W.pair(an, bn, "uniform", 9000 + k)
+        r = pg.mul(a, b)
+        if k % 37 == 0:
+            ref = pg.mul(a, b, path=pg.PATH_NTT)
+            assert np.array_equal(r, ref)
+        held, spare = pg.host_staging()
+        assert held == 0, (k, held)
+        assert spare <= cap, (k, spare)

@@ -120,3 +120,22 @@ def test_auto_dispatch_squares_and_mid_sizes(pg, oracle, n):
     assert err == 0.0 and np.array_equal(r, oracle.mul_into(a, b))
     s, serr = pg.mul(a, a, want_round=True)
     assert serr == 0.0 and np.array_equal(s, oracle.mul_into(a, a, sqr=True))
+
+
+@pytest.mark.parametrize("logn", list(range(12, 25)))
+def test_shipped_plan_every_length(pg, oracle, logn):
+    """One product per wide-mode transform length 2^12 .. 2^24 under the
+    shipped plan table (ntt_plans.txt: non-default variants at several of
+    these lengths): equal to the reference-decomposition product
+    (PG_PATH_NTT_REF, other kernels and digits), to the oracle up to 2^20
+    points, and to A B modulo three primes at every length."""
+    an, bn = (1 << (logn - 2)) - 3, 1 << (logn - 2)     # 2 (an + bn) base-2^32 digits -> n = 2^logn
+    a, b = W.pair(an, bn, "uniform", 700 + logn)
+    print("case logn", logn, "variant", pg.ntt_plans()[logn], flush=True)
+    r = pg.mul(a, b, path=pg.PATH_NTT)
+    assert np.array_equal(r, pg.mul(a, b, path=pg.PATH_NTT_REF))
+    M = oracle.MODS3
+    ra, rb, rc = oracle.mod_p3(a), oracle.mod_p3(b), oracle.mod_p3(r)
+    assert all(rc[i] == ra[i] * rb[i] % M[i] for i in range(3))
+    if logn <= 20:
+        assert np.array_equal(r, oracle.mul_ntt(a, b))

@@ -268,3 +268,46 @@ def test_against_reference_build(oracle):
         mine, err = oracle.mul_fft(a, b)
         assert np.array_equal(mine, r[:n])
         assert err == e.value        # same operation order -> same doubles
+
+
+def _is_prime64(n):
+    """Deterministic Miller-Rabin for n < 2^64 (bases 2..37)."""
+    if n < 2:
+        return False
+    small = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)
+    for q in small:
+        if n % q == 0:
+            return n == q
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in small:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def test_mod_checks_against_python_ints(oracle):
+    """or_mod_p / or_mod_p3 (the serving tests' residue checks) against
+    Python's integer remainder, on random, all-ones and empty inputs; the
+    three moduli are prime."""
+    import random
+    rng = random.Random(11)
+    for p in oracle.MODS3:
+        assert _is_prime64(p)
+    assert not _is_prime64((1 << 64) - 57)   # the check can fail
+    for n in (0, 1, 2, 3, 17, 1000):
+        for a in (np.array([rng.getrandbits(64) for _ in range(n)], np.uint64),
+                  np.full(n, 0xFFFFFFFFFFFFFFFF, np.uint64)):
+            x = int.from_bytes(a.astype("<u8").tobytes(), "little")
+            assert oracle.mod_p3(a) == tuple(x % p for p in oracle.MODS3)
+            for k, c in ((61, 1), (63, 25), (64, 59), (40, 87)):
+                assert oracle.mod_p(a, k, c) == x % ((1 << k) - c)
