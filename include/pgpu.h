@@ -90,6 +90,19 @@ uint64_t *pg_buf_ptr(pg_buf *h);
 int pg_mul_dev(pg_buf *r, size_t *r_len, const pg_buf *a, size_t an,
                const pg_buf *b, size_t bn, int path, double *fft_max_round);
 
+/* r = a * b on the NTT path (pg_mul_dev with PG_PATH_NTT) with no
+ * allocation of any kind: no reference counterpart; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  r, a and b are device pointers (the
+ * caller's own device memory, or pg_buf_ptr of library buffers); r receives
+ * an + bn limbs (the top one may be zero); an, bn are trimmed lengths.
+ * Every scratch buffer comes from one static 48 MiB device array per GPU,
+ * so the sizes are limited: 192 < max(an, bn) and an + bn <= 2^19 limbs
+ * (transforms of at most 2^20 points); other sizes fail with PG_ERANGE.
+ * Asynchronous on the library's stream (pg_sync waits); calls are ordered
+ * on that stream, so they never use the array at the same time. */
+int pg_mul_dev_static(uint64_t *r, const uint64_t *a, size_t an, const uint64_t *b, size_t bn);
+
 /*
  * Many products in one submission.  preclib has no batch interface: its
  * callers multiply one pair at a time through mul_into

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "libpgpu.so")
 SOURCES = ["pg_ntt.hip", "pg_ntt_v1.hip", "pg_ntt_v2.hip", "pg_ntt_v3.hip", "pg_ntt_v4.hip", "pg_ntt_v5.hip",
            "pg_ntt_v6.hip", "pg_ntt_v7.hip",
            "pg_ntt_v8.hip", "pg_ntt_v9.hip", "pg_ntt_v10.hip", "pg_ntt_v11.hip", "pg_ntt_v12.hip", "pg_ntt_v13.hip", "pg_ntt_v14.hip", "pg_ntt_v15.hip", "pg_fft.hip", "pg_small.hip", "pg_carry.hip", "pg_batch.hip", "pg_runtime.hip", "pg_dint.hip",
-           "pg_pi.hip", "pg_serve.hip", "pg_api.hip"]
+           "pg_pi.hip", "pg_serve.hip", "pg_noalloc.hip", "pg_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

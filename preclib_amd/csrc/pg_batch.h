@@ -84,7 +84,7 @@ private:
 };
 
 // Table pointers of a transform length (shared by single and batched launches).
-int fill_ntt_tables(NttLaunch &L);
+int fill_ntt_tables(NttLaunch &L, bool create = true);
 int fill_fft_tables(FftLaunch &L);
 
 }  // namespace pg

@@ -133,6 +133,9 @@ public:
 
     const TwTables &tw(int prime, int logB, int lr);
     const GlobalTw &gtw(int prime, int logn);
+    // the same tables if they exist, else nullptr (never allocates)
+    const TwTables *tw_find(int prime, int logB, int lr) const;
+    const GlobalTw *gtw_find(int prime, int logn) const;
     const FftTw &ftw(int logB, int lr);
     const FftGlobal &fgtw(int logn);
     FftK fk{};
@@ -207,6 +210,14 @@ int exception_status();
 // rp must hold an + bn limbs; it may alias ap or bp.  Returns a status code.
 int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,
                    const uint64_t *bp, size_t bn, uint64_t btop, int mode);
+// The launch of that product without its scratch pointers (L.one.spec, prod,
+// slo, shi): transform length, plan variant, table pointers.  create = false
+// uses only tables that already exist (PG_EINTERNAL otherwise).
+int ntt_plan(NttLaunch &L, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp, size_t bn,
+             uint64_t btop, int mode, bool create);
+// Sets up the allocation-free product (pg_mul_dev_static) on the current
+// device (pg_noalloc.hip); called by Context::init.
+int static_scratch_setup(int device);
 // mul_fft on device pointers; *max_round (if non-null) receives the largest
 // rounding distance of this product (waits for the device).
 int fft_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop,

@@ -154,6 +154,12 @@ int Context::init(int dev, size_t budget) {
     HIP_TRY(hipMalloc(&words, NWORDS * sizeof(uint64_t)));
     load_plan_table();
     if (const char *e = std::getenv("PGPU_NTT_ORDER")) ntt_order = std::atoi(e) & 7;
+    device = dev;
+    if (const int st = static_scratch_setup(dev)) {
+        const std::string msg = last_error();
+        shutdown();
+        return set_error(st, msg);
+    }
     {
         const long double PI = 3.141592653589793238462643383279502884L;
         for (int k = 0; k < 8; ++k) {
@@ -309,6 +315,16 @@ const TwTables &Context::tw(int prime, int logB, int lr) {
 // Four-step twiddles omega_n^e = lo[e mod 2^s] * hi[e >> s] in Montgomery form;
 // the inverse low table carries the scale n^-1 * 2^32 that undoes both the
 // unnormalized inverse passes and the 2^-32 of the pointwise Montgomery product.
+const TwTables *Context::tw_find(int prime, int logB, int lr) const {
+    auto it = tw_.find(((long)prime << 16) | (logB << 8) | lr);
+    return it == tw_.end() ? nullptr : &it->second;
+}
+
+const GlobalTw *Context::gtw_find(int prime, int logn) const {
+    auto it = gtw_.find(((long)prime << 16) | logn);
+    return it == gtw_.end() ? nullptr : &it->second;
+}
+
 const GlobalTw &Context::gtw(int prime, int logn) {
     const long key = ((long)prime << 16) | logn;
     auto it = gtw_.find(key);
@@ -507,30 +523,36 @@ int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *o
     return PG_OK;
 }
 
-int fill_ntt_tables(NttLaunch &L) {
+int fill_ntt_tables(NttLaunch &L, bool create) {
     Context &C = ctx();
     const int logn = L.logn, NP = mode_primes(L.mode);
     L.nprimes = NP;
     L.crt = C.crt;
     const int v = L.mode == MODE_WIDE ? L.variant : 0;
     const int l1 = plan_l1(logn, v), l2 = plan_l2(logn, v), rb = plan_rb(logn, v);
+    auto pass = [&](int k, int logB, int lr) -> const TwTables * {
+        return create ? &C.tw(k, logB, lr) : C.tw_find(k, logB, lr);
+    };
     for (int k = 0; k < NP; ++k) {
         L.pk[k] = C.pk[k];
         int logB = l1;
         for (int i = 0; i < split_np(l1, rb); ++i) {
-            const TwTables &t = C.tw(k, logB, split_lr(l1, i, rb));
-            L.col_tw[k][i] = t.fwd; L.col_itw[k][i] = t.inv;
+            const TwTables *t = pass(k, logB, split_lr(l1, i, rb));
+            if (!t) return set_error(PG_EINTERNAL, "twiddle tables not prepared");
+            L.col_tw[k][i] = t->fwd; L.col_itw[k][i] = t->inv;
             logB -= split_lr(l1, i, rb);
         }
         logB = l2;
         for (int i = 0; i < split_np(l2, rb); ++i) {
-            const TwTables &t = C.tw(k, logB, split_lr(l2, i, rb));
-            L.row_tw[k][i] = t.fwd; L.row_itw[k][i] = t.inv;
+            const TwTables *t = pass(k, logB, split_lr(l2, i, rb));
+            if (!t) return set_error(PG_EINTERNAL, "twiddle tables not prepared");
+            L.row_tw[k][i] = t->fwd; L.row_itw[k][i] = t->inv;
             logB -= split_lr(l2, i, rb);
         }
-        const GlobalTw &g = C.gtw(k, logn);
-        if (!g.glo) return set_error(PG_ENOMEM, "twiddle tables");
-        L.glo[k] = g.glo; L.ghi[k] = g.ghi; L.ilo[k] = g.ilo; L.ihi[k] = g.ihi; L.lobits = g.lobits;
+        const GlobalTw *g = create ? &C.gtw(k, logn) : C.gtw_find(k, logn);
+        if (!g) return set_error(PG_EINTERNAL, "twiddle tables not prepared");
+        if (!g->glo) return set_error(PG_ENOMEM, "twiddle tables");
+        L.glo[k] = g->glo; L.ghi[k] = g->ghi; L.ilo[k] = g->ilo; L.ihi[k] = g->ihi; L.lobits = g->lobits;
     }
     return PG_OK;
 }
@@ -573,8 +595,8 @@ int launch_carry_nodes(size_t nnodes, double bytes, const CarryNode *dnodes, con
     return PG_OK;
 }
 
-int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,
-                   size_t bn, uint64_t btop, int mode) {
+int ntt_plan(NttLaunch &L, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp, size_t bn,
+             uint64_t btop, int mode, bool create) {
     Context &C = ctx();
     if (!C.stream) return set_error(PG_ENOINIT, "pg_init not called");
     if (!an || !bn || !atop || !btop) return set_error(PG_EINVAL, "operands must be trimmed and nonzero");
@@ -593,17 +615,24 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     while (((size_t)1 << logn) < da + db) ++logn;
     if (logn < NTT_MIN_LOG) logn = NTT_MIN_LOG;
     if (logn > NTT_MAX_LOG) return set_error(PG_ERANGE, "transform too long");
-    const size_t n = (size_t)1 << logn;
-    const int NP = mode_primes(mode), DPL = 64 / mode_digit_bits(mode);
-    Scratch s;
-    NttLaunch L{};
+    L = NttLaunch{};
     L.one.a = ap; L.one.b = bp; L.one.an = an; L.one.bn = bn;
     L.io_bytes = (double)(an + bn) * 16.0;
     L.logn = logn; L.mode = mode;
     L.variant = C.ntt_variant(logn, mode);
     L.order = C.ntt_order;
-    { int st = fill_ntt_tables(L); if (st) return st; }
-    const int l1 = plan_l1(logn, mode == MODE_WIDE ? L.variant : 0);
+    return fill_ntt_tables(L, create);
+}
+
+int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, const uint64_t *bp,
+                   size_t bn, uint64_t btop, int mode) {
+    Context &C = ctx();
+    NttLaunch L;
+    { int st = ntt_plan(L, ap, an, atop, bp, bn, btop, mode, true); if (st) return st; }
+    const size_t n = (size_t)1 << L.logn;
+    const int NP = mode_primes(mode), DPL = 64 / mode_digit_bits(mode);
+    Scratch s;
+    const int l1 = plan_l1(L.logn, mode == MODE_WIDE ? L.variant : 0);
     if (l1) L.one.spec = s.get<uint32_t>(2 * NP * n);
     L.one.prod = s.get<uint32_t>(NP * n);
     const size_t m = n / DPL;

@@ -25,7 +25,8 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev", "pg_sqrt", "pg_to_decimal",
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
            "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
-           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_host_staging"]
+           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_host_staging",
+           "pg_mul_dev_static"]
 NTT_VARIANTS = 16
 OP_MUL, OP_DIV = 0, 1
 
@@ -211,6 +212,13 @@ def mul_dev(r, a, an, b, bn, path=PATH_AUTO, want_len=True):
 
 def sync():
     _check(_lib.pg_sync())
+
+
+def mul_dev_static(r, a, an, b, bn):
+    """pg_mul_dev_static on DevBuf handles' device pointers (asynchronous)."""
+    L = load()
+    L.pg_mul_dev_static.argtypes = [ctypes.c_void_p, ctypes.c_void_p, SZ, ctypes.c_void_p, SZ]
+    _check(L.pg_mul_dev_static(L.pg_buf_ptr(r.h), L.pg_buf_ptr(a.h), an, L.pg_buf_ptr(b.h), bn))
 
 
 def host_staging():
