@@ -166,6 +166,14 @@ int pg_sqrt(uint64_t *x, size_t x_cap, size_t *x_len, const uint64_t *a, size_t 
  * an * 19.27 + 1 characters. */
 int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t an);
 
+/* precn_t(std::string) (src/init_basic.cpp:66-92): the value of a decimal
+ * string of n characters, most significant digit first; every character
+ * outside '0'..'9' is skipped, as the reference does.  out receives *len
+ * trimmed limbs (0 for zero); it needs room for n * log2(10) / 64 + 1 limbs.
+ * Divide and conquer on the GPU: 19-digit chunks, then node = hi * 10^k + lo
+ * level by level with batched products (the inverse of pg_to_decimal). */
+int pg_from_decimal(uint64_t *out, size_t cap, size_t *len, const char *s, size_t n);
+
 /* chud_bs(0, nterms, need_p) (pi/pi_chudnovsky.cpp:117-148): P, Q and T of
  * the Chudnovsky series over terms [0, nterms), nterms <= 2^20.  T is
  * returned as its magnitude (it is positive for every nterms >= 1).  p may

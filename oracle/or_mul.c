@@ -161,3 +161,14 @@ void or_mod_p3(const uint64_t *a, size_t n, uint64_t out[3]) {
     }
     out[0] = r0; out[1] = r1; out[2] = r2;
 }
+
+void or_dec_mod_p3(const char *s, size_t n, uint64_t out[3]) {
+    uint64_t r0 = 0, r1 = 0, r2 = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t d = (uint64_t)(s[i] - '0');
+        r0 = fold((u128)r0 * 10 + d, 61, 1);
+        r1 = fold((u128)r1 * 10 + d, 63, 25);
+        r2 = fold((u128)r2 * 10 + d, 64, 59);
+    }
+    out[0] = r0; out[1] = r1; out[2] = r2;
+}

@@ -91,6 +91,10 @@ uint64_t or_mod_p(const uint64_t *a, size_t n, int k, uint64_t c);
 /* The same for the three moduli 2^61 - 1, 2^63 - 25 and 2^64 - 59 in one
  * pass (independent chains); out[3]. */
 void or_mod_p3(const uint64_t *a, size_t n, uint64_t out[3]);
+/* The value of the decimal digits s[0, n) ('0'..'9', most significant
+ * first) modulo the same three moduli, by Horner's rule on the digits;
+ * checks the decimal parse without a second parser. */
+void or_dec_mod_p3(const char *s, size_t n, uint64_t out[3]);
 
 /* ---- synthetic operands ------------------------------------------------ */
 /* splitmix64 stream (Vigna's published generator). dist: 0 uniform, 1 all

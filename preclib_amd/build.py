@@ -58,7 +58,8 @@ def build_host(verbose=False):
     program (which also links the oracle, as its checker)."""
     inc = os.path.join(ROOT, "include")
     _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + inc, os.path.join(HERE, "host", "precn.cpp"),
-          "-L" + HERE, "-lpgpu", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB], verbose)
+          os.path.join(HERE, "host", "precz.cpp"), "-L" + HERE, "-lpgpu", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB],
+         verbose)
     orc = os.path.join(ROOT, "oracle")
     _run([CXX, "-O2", "-std=c++17", "-I" + inc, "-I" + orc, os.path.join(ROOT, "tests", "cpp", "test_precn.cpp"),
           "-L" + HERE, "-lprecn", "-lpgpu", "-L" + orc, "-loracle",

@@ -413,6 +413,26 @@ int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t 
     return pg::exception_status();
 }
 
+int pg_from_decimal(uint64_t *out, size_t cap, size_t *len, const char *s, size_t n) try {
+    LOCK;
+    NEED_INIT;
+    if ((n && !s) || !len) return set_error(PG_EINVAL, "pg_from_decimal: bad argument");
+    // every character outside '0'..'9' is skipped (init_basic.cpp:69-72)
+    std::string digits;
+    digits.reserve(n);
+    for (size_t i = 0; i < n; ++i)
+        if (s[i] >= '0' && s[i] <= '9') digits.push_back(s[i]);
+    std::vector<uint64_t> v;
+    const int st = from_decimal(digits.data(), digits.size(), v);
+    if (st) return st;
+    *len = v.size();
+    if (v.size() > cap || (!out && !v.empty())) return set_error(PG_EINVAL, "pg_from_decimal: output capacity too small");
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * 8);
+    return PG_OK;
+} catch (...) {
+    return pg::exception_status();
+}
+
 int pg_chud_split(uint64_t *p, size_t p_cap, size_t *p_len, uint64_t *q, size_t q_cap, size_t *q_len, uint64_t *t,
                   size_t t_cap, size_t *t_len, size_t nterms) try {
     LOCK;

@@ -17,6 +17,10 @@ int isqrt(DBuf &x, size_t &xl, const uint64_t *a, size_t an);
 // Decimal digits of x, most significant first, no leading zeros; empty for
 // zero (precn_base_convert with base 10, src/base_convert.cpp:107-172).
 int to_decimal(const uint64_t *x, size_t xn, std::string &out);
+// The value of a string of n decimal digits ('0'..'9' only, most significant
+// first), trimmed limbs (empty for zero): precn_t(std::string),
+// src/init_basic.cpp:66-92.
+int from_decimal(const char *digits, size_t n, std::vector<uint64_t> &out);
 // The output of pi_chudnovsky DIGITS (pi_chudnovsky.cpp:162-193) without the
 // trailing newline; phase_ms (if set) gets binary_split, sqrt_scale,
 // final_div and to_decimal times.

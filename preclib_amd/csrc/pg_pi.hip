@@ -799,6 +799,145 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
     return PG_OK;
 }
 
+// ---- decimal parse (precn_t(std::string), src/init_basic.cpp:7-45, 66-92) -------------
+// The inverse of to_decimal's tree: the digit string is cut into chunks of 19
+// digits from its least significant end, DEC_LEAF chunks to a leaf, and
+// leaf / node values combine pairwise, level by level:
+//   node_(i+1)[k] = node_i[2k + 1] * U_i + node_i[2k],  U_i = 10^(19 DEC_LEAF 2^i),
+// with U_(i+1) = U_i^2.  A node of level i is below U_i, so it fits the
+// limbs of U_i; every level is one batched product launch (all nodes times
+// the same U_i) and one linear launch for the sums.  The reference combines
+// chunks of 9 digits with the same halving recursion (init_basic.cpp:30-45);
+// the value is the same.
+
+// One thread per leaf: chunks [DEC_LEAF i, DEC_LEAF (i + 1)) of the string
+// (chunk c is digits [n - 19 (c + 1), n - 19 c), the first one possibly
+// shorter) by Horner's rule in base 10^19, into out + DEC_LEAF i.
+__global__ void __launch_bounds__(64) k_dec_parse_leaf(const char *digits, uint64_t n, uint64_t nchunk,
+                                                       uint32_t nleaf, uint64_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaf) return;
+    uint64_t acc[DEC_LEAF];
+#pragma unroll
+    for (int k = 0; k < DEC_LEAF; ++k) acc[k] = 0;
+    const uint64_t c0 = (uint64_t)i * DEC_LEAF;
+    for (int c = DEC_LEAF - 1; c >= 0; --c) {
+        const uint64_t ch = c0 + c;
+        if (ch >= nchunk) continue;
+        const uint64_t end = n - 19 * ch, beg = end > 19 ? end - 19 : 0;
+        uint64_t v = 0;
+        for (uint64_t p = beg; p < end; ++p) v = v * 10 + (uint64_t)(digits[p] - '0');
+        // acc = acc * 10^19 + v
+        uint64_t carry = v;
+#pragma unroll
+        for (int k = 0; k < DEC_LEAF; ++k) {
+            const unsigned __int128 t = (unsigned __int128)acc[k] * TEN19 + carry;
+            acc[k] = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < DEC_LEAF; ++k) out[c0 + k] = acc[k];
+}
+
+int from_decimal(const char *digits, size_t n, std::vector<uint64_t> &out) {
+    Context &C = ctx();
+    out.clear();
+    size_t lead = 0;
+    while (lead < n && digits[lead] == '0') ++lead;
+    digits += lead;
+    n -= lead;
+    if (!n) return PG_OK;
+    int st;
+    const size_t nchunk = (n + 18) / 19, nleaf = (nchunk + DEC_LEAF - 1) / DEC_LEAF;
+    int top = 0;
+    while (((size_t)1 << top) < nleaf) ++top;
+    // U_0 = 10^(19 DEC_LEAF) on the host; its limbs bound a leaf
+    std::vector<uint64_t> u0(1, 1);
+    for (int k = 0; k < DEC_LEAF; ++k) {
+        uint64_t carry = 0;
+        for (auto &w : u0) {
+            const unsigned __int128 t = (unsigned __int128)w * TEN19 + carry;
+            w = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+        if (carry) u0.push_back(carry);
+    }
+    if (u0.size() != (size_t)DEC_LEAF) return set_error(PG_EINTERNAL, "decimal parse: leaf bound");
+    DBuf dstr((n + 7) / 8), leaves(nleaf * DEC_LEAF);
+    if (!dstr.d || !leaves.d) return set_error(PG_ENOMEM, "decimal parse");
+    {
+        HostImg img = C.take_img(n);
+        std::memcpy(img.data(), digits, n);
+        HIP_TRY(hipMemcpyAsync(dstr.d, img.data(), n, hipMemcpyHostToDevice, C.stream));
+        C.hold(std::move(img));
+    }
+    hipEvent_t e0 = nullptr;
+    C.stage_begin(e0);
+    hipLaunchKernelGGL(k_dec_parse_leaf, dim3(blocks_for(nleaf, 64)), dim3(64), 0, C.stream, (const char *)dstr.d,
+                       (uint64_t)n, (uint64_t)nchunk, (uint32_t)nleaf, leaves.d);
+    C.stage_end("decimal_parse_leaf", e0, (double)n + 8.0 * DEC_LEAF * nleaf);
+    HIP_TRY(hipGetLastError());
+    // node k of the current level: limbs [p, p + len) (len = the level's bound)
+    std::vector<const uint64_t *> nodes(nleaf);
+    for (size_t k = 0; k < nleaf; ++k) nodes[k] = leaves.d + k * DEC_LEAF;
+    size_t len = DEC_LEAF;
+    DBuf U(DEC_LEAF);
+    if (!U.d || (st = dset(U.d, DEC_LEAF, u0.data(), DEC_LEAF))) return st ? st : set_error(PG_ENOMEM, "decimal parse");
+    size_t ul = DEC_LEAF;   // limbs of U_i (its top limb is nonzero)
+    std::vector<DBuf> keep;
+    for (int i = 0; i < top; ++i) {
+        const size_t nm = nodes.size() / 2, np = (nodes.size() + 1) / 2;
+        // products hi * U_i (len + ul limbs) and the level's nodes (2 len limbs each)
+        const size_t nlen = len + ul;
+        keep.emplace_back(nm * nlen + np * nlen);
+        if (!keep.back().d) return set_error(PG_ENOMEM, "decimal parse");
+        uint64_t *prods = keep.back().d, *lvl = prods + nm * nlen;
+        std::vector<pg_job> jobs(nm);
+        std::vector<CarryNode> sums(nm);
+        std::vector<const uint64_t *> up(np);
+        for (size_t k = 0; k < nm; ++k) {
+            pg_job &j = jobs[k];
+            j = pg_job{};
+            j.a = nodes[2 * k + 1]; j.an = len; j.b = U.d; j.bn = ul;
+            j.r = prods + k * nlen; j.r_cap = nlen; j.path = PG_PATH_AUTO;
+            CarryNode c = lin_node(lvl + k * nlen, nlen);
+            c.x = prods + k * nlen; c.xn = nlen; c.y = nodes[2 * k]; c.yn = len;
+            sums[k] = c;
+            up[k] = lvl + k * nlen;
+        }
+        if (np > nm) {   // the most significant node has no partner: zero-extend it
+            CarryNode c = lin_node(lvl + nm * nlen, nlen);
+            c.x = nodes.back(); c.xn = len;
+            sums.push_back(c);
+            up[nm] = lvl + nm * nlen;
+        }
+        if (nm) {
+            Batch b;
+            b.use_graph = false;
+            if ((st = b.build(jobs.data(), nm)) || (st = b.run())) return st;
+        }
+        if ((st = lin_run(sums.data(), sums.size(), "decimal_parse"))) return st;
+        nodes.swap(up);
+        len = nlen;
+        if (i + 1 < top) {   // U_(i+1) = U_i^2
+            DBuf U2(2 * ul);
+            if (!U2.d) return set_error(PG_ENOMEM, "decimal parse");
+            if ((st = dmul(U2.d, U.d, ul, U.d, ul))) return st;
+            size_t ul2 = 0;
+            if ((st = top_len(U2.d, 2 * ul, &ul2))) return st;
+            keep.push_back(std::move(U));
+            U = std::move(U2);
+            ul = ul2;
+        }
+    }
+    size_t rl = 0;
+    if ((st = top_len(nodes[0], len, &rl))) return st;
+    out.resize(rl);
+    if (rl) HIP_TRY(hipMemcpyAsync(out.data(), nodes[0], rl * 8, hipMemcpyDeviceToHost, C.stream));
+    return C.sync();
+}
+
 // ---- the pi program ------------------------------------------------------------------
 
 int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {

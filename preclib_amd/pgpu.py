@@ -26,7 +26,7 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
            "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
            "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_host_staging",
-           "pg_mul_dev_static"]
+           "pg_mul_dev_static", "pg_from_decimal"]
 NTT_VARIANTS = 16
 OP_MUL, OP_DIV = 0, 1
 
@@ -219,6 +219,20 @@ def mul_dev_static(r, a, an, b, bn):
     L = load()
     L.pg_mul_dev_static.argtypes = [ctypes.c_void_p, ctypes.c_void_p, SZ, ctypes.c_void_p, SZ]
     _check(L.pg_mul_dev_static(L.pg_buf_ptr(r.h), L.pg_buf_ptr(a.h), an, L.pg_buf_ptr(b.h), bn))
+
+
+def from_decimal(s):
+    """The integer of a decimal string as limbs (pg_from_decimal)."""
+    L = load()
+    if not _inited:
+        init()
+    b = s.encode() if isinstance(s, str) else bytes(s)
+    cap = len(b) * 3322 // 64000 + 2
+    out = np.zeros(max(cap, 1), np.uint64)
+    n = SZ(0)
+    L.pg_from_decimal.argtypes = [P64, SZ, ctypes.POINTER(SZ), ctypes.c_char_p, SZ]
+    _check(L.pg_from_decimal(_ptr(out), cap, ctypes.byref(n), b, len(b)))
+    return out[: n.value].copy()
 
 
 def host_staging():

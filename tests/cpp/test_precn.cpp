@@ -156,8 +156,133 @@ static void test_sqrt_and_text() {
     expect(nd == 4 && d[0] == 0 && d[1] == 0xF && d[3] == 0xF, "precn_base_convert base 16");
 }
 
+static void test_linear_and_scalar() {
+    // sub.cpp:14: a - b saturates to 0 when a < b
+    precn_t a = fill(300, 901), b = fill(301, 902);
+    expect((a - b).rsiz == 0 && (precn_t(3) - precn_t(7)).rsiz == 0, "a - b saturates");
+    expect(b - a + a == b, "b - a + a");
+    precn_t z;
+    expect(--z == precn_t() && z.rsiz == 0, "--0 stays 0");
+    precn_t c(41);
+    expect(c++ == precn_t(41) && c == precn_t(42) && ++c == precn_t(43), "increments");
+    expect(c-- == precn_t(43) && --c == precn_t(41), "decrements");
+    precn_t ff = limbs({~0ull, ~0ull});
+    expect(++ff == limbs({0, 0, 1}), "++ carries");
+    expect((limbs({0xF0F0, 0xFF}) & limbs({0xFF00})) == precn_t(0xF000), "and");
+    expect((limbs({0xF0F0}) | limbs({0x0F0F, 1})) == limbs({0xFFFF, 1}), "or");
+    expect((limbs({5, 7}) ^ limbs({5, 7})).rsiz == 0, "xor trims");
+    expect(+a == a, "unary plus");
+    // scalar division by one word (div_basic.cpp:135-198); by 0 -> 0
+    std::vector<uint64_t> q(300), r(2);
+    size_t ql = 0, rl = 0;
+    const uint64_t dw = 0xFEDCBA9876543211ull;
+    or_divmod(q.data(), q.size(), &ql, r.data(), r.size(), &rl, a.a, a.rsiz, &dw, 1);
+    expect(same(div_u64(a, dw), q, ql) && same(mod_u64(a, dw), r, rl), "div_u64 / mod_u64");
+    expect(same(a / dw, q, ql) && same(a % dw, r, rl), "operator/ % by a word");
+    expect(div_u64(a, 0).rsiz == 0 && mod_u64(a, 0).rsiz == 0, "by zero word");
+    expect(div_u32(limbs({~0ull}), 2) == limbs({0x7FFFFFFFFFFFFFFFull}), "div_u32");
+    expect(mul_u64(a, 3) == a * 3 && a * 3 == 3 * a && a + 5 == 5 + a, "scalar templates");
+    expect(precn_t(100) / precn_t(7) == precn_t(14) && 100 / precn_t(7) == precn_t(14) &&
+               100 % precn_t(7) == precn_t(2), "scalar left operands");
+    // exact division helpers (mul_toom.cpp:38-70)
+    precn_t t3 = mul_u64(a, 3);
+    expect(precn_divexact_3(t3) == a && precn_divexact_2(a + a) == a, "divexact");
+}
+
+static void test_division_family() {
+    // div_basic.cpp:318-407: every entry gives floor(a / b) and a mod b; a
+    // zero operand gives 0; q and r may alias the operands
+    for (size_t an : {1, 2, 40, 300, 5000})
+        for (size_t bn : {1, 2, 3, 64, 200, 4000}) {
+            precn_t a = fill(an, 7000 + an), b = fill(bn, 8000 + bn);
+            std::vector<uint64_t> q(an + 2), r(an + bn + 2);
+            size_t ql = 0, rl = 0;
+            or_divmod(q.data(), q.size(), &ql, r.data(), r.size(), &rl, a.a, a.rsiz, b.a, b.rsiz);
+            precn_t x, y;
+            div_into(x, a, b);
+            mod_into(y, a, b);
+            expect(same(x, q, ql) && same(y, r, rl), "div_into / mod_into", an, bn);
+            divmod_into(x, y, a, b);
+            expect(same(x, q, ql) && same(y, r, rl), "divmod_into", an, bn);
+            divmod_schoolbook_into(x, y, a, b);
+            expect(same(x, q, ql) && same(y, r, rl), "divmod_schoolbook_into", an, bn);
+            expect(same(div_schoolbook(a, b), q, ql) && same(mod_schoolbook(a, b), r, rl), "schoolbook", an, bn);
+            precn_t aa = a;
+            div_into(aa, aa, b);
+            expect(same(aa, q, ql), "div_into aliased", an, bn);
+            aa = a;
+            mod_into(aa, aa, b);
+            expect(same(aa, r, rl), "mod_into aliased", an, bn);
+            precn_t dq, dr;
+            const bool dc = div_dc_into(dq, dr, a, b);
+            const bool want_dc = bn >= 64 && !(a < b) && an <= 2 * bn;   // div_dc.cpp:93-94
+            expect(dc == want_dc && (!dc || (same(dq, q, ql) && same(dr, r, rl))), "div_dc_into", an, bn);
+        }
+    precn_t a = fill(50, 1);
+    precn_t q1, r1;
+    divmod_into(q1, r1, a, precn_t());
+    expect(q1.rsiz == 0 && r1.rsiz == 0 && (a / precn_t()).rsiz == 0 && (a % precn_t()).rsiz == 0, "by zero");
+    precn_t g = gcd(fill(40, 5) * fill(30, 6), fill(40, 5) * fill(20, 9));
+    expect(g % fill(40, 5) == precn_t() && gcd(precn_t(12), precn_t(18)) == precn_t(6) &&
+               gcd(precn_t(), precn_t(5)) == precn_t(5), "gcd");
+}
+
+static void test_parse_and_bases() {
+    // init_basic.cpp:66-92: digits only, other characters skipped
+    expect(precn_t(std::string("4294967296")) == limbs({1ull << 32}), "parse 2^32");
+    expect(precn_t(std::string("12a34")) == precn_t(1234), "parse skips letters");
+    expect(precn_t(std::string("")).rsiz == 0 && precn_t(std::string("000")).rsiz == 0 &&
+               precn_t(std::string("-+x")).rsiz == 0, "parse empty / zeros");
+    for (size_t n : {1, 5, 100, 3000, 40000}) {
+        precn_t a = fill(n, 31 + n);
+        const std::string s = (std::string)a;
+        expect(precn_t(s) == a, "decimal round trip", n);
+        expect(precn_t("00" + s) == a, "leading zeros", n);
+    }
+    // other bases (base_convert.cpp:107-172), least significant digit first
+    for (uint32_t base : {3u, 7u, 10u, 16u, 1000u, 4294967291u}) {
+        precn_t a = fill(60, 90 + base);
+        size_t nd = 0;
+        precn_base_convert(a, base, nullptr, nd);
+        std::vector<uint32_t> d(nd + 1);
+        size_t nd2 = 0;
+        precn_base_convert(a, base, d.data(), nd2);
+        precn_t back;
+        for (size_t i = nd2; i-- > 0;) back = mul_u64(back, base) + precn_t(d[i]);
+        expect(nd == nd2 && nd && d[nd - 1] != 0 && back == a, "base round trip", base);
+    }
+}
+
+static void test_signed() {
+    // precz.cpp semantics over every sign: truncating quotient, remainder
+    // with the dividend's sign (as C++'s int64 / and %), zero never negative
+    for (long x = -100; x <= 100; ++x)
+        for (long y = -100; y <= 100; ++y) {
+            const precz_t a(x), b(y);
+            bool ok = (long)(a + b) == x + y && (long)(a - b) == x - y && (long)(a * b) == x * y;
+            ok = ok && ((a < b) == (x < y)) && ((a == b) == (x == y)) && ((a >= b) == (x >= y));
+            if (y) ok = ok && (long)(a / b) == x / y && (long)(a % b) == x % y;
+            else ok = ok && (a / b).is_zero() && (a % b).is_zero();
+            ok = ok && !(a * precz_t(0)).is_negative() && (a - a).is_zero() && !(a - a).is_negative();
+            expect(ok, "precz arithmetic", (size_t)(x + 100), (size_t)(y + 100));
+        }
+    expect((std::string)precz_t(std::string("  -123")) == "-123" && (std::string)precz_t(std::string("-0")) == "0" &&
+               (std::string)precz_t(-45) == "-45" && (long)precz_t(std::string("77")) == 77, "precz text");
+    precz_t big = precz_t(std::string("-123456789012345678901234567890"));
+    expect((std::string)(big * big) == "15241578753238836750495351562536198787501905199875019052100", "precz big");
+    expect(abs(big) == -big && (precz_t(-8) << 2) == precz_t(-32) && (precz_t(-33) >> 2) == precz_t(-8) &&
+               gcd(precz_t(-12), precz_t(18)) == precz_t(6) && precz_sqrt(precz_t(-4)).is_zero() &&
+               precz_sqrt(precz_t(17)) == precz_t(4), "precz helpers");
+    precz_t i(-1);
+    expect((i++ == precz_t(-1)) && i.is_zero() && (--i == precz_t(-1)), "precz increments");
+}
+
 int main() {
     precn_gpu_init(0);
+    test_linear_and_scalar();
+    test_division_family();
+    test_parse_and_bases();
+    test_signed();
     test_known_answers();
     test_all_mul_eq();
     test_dispatcher_shapes();

@@ -76,6 +76,7 @@ class Oracle:
         L.or_mod_p.restype = ctypes.c_uint64
         L.or_mod_p.argtypes = [P64, SZ, ctypes.c_int, ctypes.c_uint64]
         L.or_mod_p3.argtypes = [P64, SZ, P64]
+        L.or_dec_mod_p3.argtypes = [ctypes.c_char_p, SZ, P64]
 
     def _out(self, an, bn):
         return np.zeros(max(an + bn, 1), np.uint64)
@@ -186,6 +187,13 @@ class Oracle:
         a = np.ascontiguousarray(a, dtype=np.uint64)
         out = np.zeros(3, np.uint64)
         self.lib.or_mod_p3(_p64(a), len(a), _p64(out))
+        return tuple(int(v) for v in out)
+
+    def dec_mod_p3(self, s):
+        """A decimal digit string's value mod the MODS3 moduli."""
+        b = s.encode() if isinstance(s, str) else bytes(s)
+        out = np.zeros(3, np.uint64)
+        self.lib.or_dec_mod_p3(b, len(b), _p64(out))
         return tuple(int(v) for v in out)
 
     def fill(self, n, seed, dist):

@@ -311,3 +311,18 @@ def test_mod_checks_against_python_ints(oracle):
             assert oracle.mod_p3(a) == tuple(x % p for p in oracle.MODS3)
             for k, c in ((61, 1), (63, 25), (64, 59), (40, 87)):
                 assert oracle.mod_p(a, k, c) == x % ((1 << k) - c)
+
+
+def test_decimal_residues_against_python_ints(oracle):
+    """or_dec_mod_p3 (the decimal parse's residue check) against Python's
+    own integer parse and remainder."""
+    import random
+    import sys
+    if hasattr(sys, "set_int_max_str_digits"):
+        sys.set_int_max_str_digits(0)
+    rng = random.Random(5)
+    for n in (0, 1, 2, 19, 20, 700, 5000):
+        s = "".join(rng.choice("0123456789") for _ in range(n))
+        x = int(s) if s else 0
+        assert oracle.dec_mod_p3(s) == tuple(x % p for p in oracle.MODS3)
+    assert oracle.dec_mod_p3("9" * 3000) == tuple((10 ** 3000 - 1) % p for p in oracle.MODS3)
