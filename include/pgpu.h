@@ -245,6 +245,16 @@ int pg_sync(void);
  * the device recycles the held images. */
 int pg_host_staging(size_t *held_bytes, size_t *spare_bytes);
 
+/* Test hook, no counterpart in the reference: perturbs the next division
+ * (pg_div, pg_div_dev, pg_recip and the divisions inside pg_sqrt / pg_pi)
+ * so that its corrections are exercised.  what 0: the first quotient
+ * estimate q = floor(a x / 2^scale) is moved by `amount` units; what 1: the
+ * Newton seed of the next reciprocal is multiplied by `amount` (an estimate
+ * too large, which takes the halving branch of div_newton.cpp:57-62).  The
+ * results stay exact: the quotient correction (div_newton.cpp:114-125)
+ * removes any error.  One-shot; 0 clears. */
+int pg_debug_perturb(int what, int64_t amount);
+
 /* Per-stage kernel timing on the library's stream (HIP events around each
  * launch).  pg_timing_read fills up to `cap` entries and returns the number of
  * stages; names are stable identifiers such as "ntt_col_fwd". */

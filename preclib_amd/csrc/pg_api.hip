@@ -509,6 +509,14 @@ int pg_timing_reset(void) try {
     return pg::exception_status();
 }
 
+int pg_debug_perturb(int what, int64_t amount) try {
+    LOCK;
+    NEED_INIT;
+    return pg::debug_perturb(what, amount);
+} catch (...) {
+    return pg::exception_status();
+}
+
 int pg_host_staging(size_t *held_bytes, size_t *spare_bytes) try {
     LOCK;
     size_t sp = 0;

@@ -65,5 +65,9 @@ int recip_finish(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits)
 // a, b trimmed; q gets ql trimmed limbs, r (if non-null) rl trimmed limbs.
 int div_mulinv(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size_t an, const uint64_t *b,
                size_t bn);
+// Test hook: what 0 adds `amount` units to the next division's first
+// quotient estimate; what 1 multiplies the next reciprocal's seed by
+// `amount` (an estimate too large, which takes the overshoot branch).
+int debug_perturb(int what, int64_t amount);
 
 }  // namespace pg

@@ -203,7 +203,39 @@ std::vector<uint64_t> pow2_div_small(size_t e, const std::vector<uint64_t> &d) {
 
 // ---- Newton reciprocal and division ---------------------------------------------
 
+// Test hooks (pg_debug_perturb): a one-shot change of the next division's
+// first quotient estimate, and a factor on the next reciprocal's seed.
+static int64_t g_perturb_q = 0;
+static uint64_t g_perturb_seed = 0;
+
+int debug_perturb(int what, int64_t amount) {
+    if (what == 0) g_perturb_q = amount;
+    else if (what == 1 && amount >= 0) g_perturb_seed = (uint64_t)amount;
+    else return set_error(PG_EINVAL, "pg_debug_perturb: bad argument");
+    return PG_OK;
+}
+
+// x = the reciprocal estimate of div_newton.cpp:36-68.  The fast form reads
+// no word from the device inside the loop: a floor estimate never takes
+// the overshoot branch (div_newton.cpp:57-62), and each step's sign word is
+// read once at the end.  If one shows an overshoot the iteration runs
+// again in the checked form, which reads every step's test and halves the
+// scaled estimate as the reference does.
+static int recip_run(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, size_t bits, bool checked,
+                     uint64_t seed_mul, bool &overshoot);
+
 int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, size_t bits) {
+    const uint64_t seed_mul = g_perturb_seed;
+    g_perturb_seed = 0;
+    bool over = false;
+    int st = recip_run(x, xl, b, bn, nb, bits, false, seed_mul, over);
+    if (st || !over) return st;
+    return recip_run(x, xl, b, bn, nb, bits, true, seed_mul, over);
+}
+
+static int recip_run(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, size_t bits, bool checked,
+                     uint64_t seed_mul, bool &overshoot) {
+    overshoot = false;
     xl = 0;
     if (!bn || !nb || bits < nb) return PG_OK;   // div_newton.cpp:37
     const size_t target = bits - nb;
@@ -228,8 +260,17 @@ int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, s
         trim(bt);
     }
     const size_t precision = keep + known;
-    const std::vector<uint64_t> x0 = pow2_div_small(precision, bt);
-    xl = limbs_for_bits(known + 2) + 1;
+    std::vector<uint64_t> x0 = pow2_div_small(precision, bt);
+    if (seed_mul > 1) {   // test hook: a seed seed_mul times too large
+        unsigned __int128 c = 0;
+        for (auto &w : x0) {
+            c += (unsigned __int128)w * seed_mul;
+            w = (uint64_t)c;
+            c >>= 64;
+        }
+        if (c) x0.push_back((uint64_t)c);
+    }
+    xl = std::max(limbs_for_bits(known + 2) + 1, x0.size());
     if (!x.reset(xl)) return set_error(PG_ENOMEM, "reciprocal");
     int st = dset(x.d, xl, x0.data(), x0.size());
     if (st) return st;
@@ -252,14 +293,31 @@ int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, s
             if ((st = lin_one(c, "newton"))) return st;
             NB = nbuf.d;
         }
-        const size_t next = nkeep + nk, s = nk - known;
+        const size_t next = nkeep + nk;
+        size_t s = nk - known;
         if (!bx.reset(nbl + xl)) return set_error(PG_ENOMEM, "reciprocal");
         if ((st = dmul(bx.d, NB, nbl, x.d, xl))) return st;
-        // e = 2^(next+1) - (bx << s); its sign word must say e >= 0 (the
-        // overshoot branch of div_newton.cpp:57-62 cannot be taken from a
-        // floor estimate)
         const size_t el = limbs_for_bits(next + 2);
         const size_t em = std::max(el, nbl + xl + limbs_for_bits(s) + 1);
+        if (checked) {
+            // div_newton.cpp:57-62: if next_b (x << s) >= 2^(next+1), the scaled
+            // estimate is halved, x << (s - 1): test 2^(next+1) - 1 - (bx << s) >= 0
+            if (!e.reset(em)) return set_error(PG_ENOMEM, "reciprocal");
+            CarryNode t = lin_node(e.d, em);
+            t.pow2 = (int64_t)(next + 1);
+            t.y = bx.d;
+            t.yn = nbl + xl;
+            t.ysh = -(int64_t)s;
+            t.yneg = 1;
+            t.flag = flags.d;
+            uint64_t ok = 0;
+            if ((st = lin_one(t, "newton")) || (st = read_words(flags.d, 1, &ok))) return st;
+            if (!ok) {
+                overshoot = true;
+                --s;   // s >= 1: nk > known
+            }
+        }
+        // e = 2^(next+1) - (bx << s) >= 0
         if (!e.reset(el)) return set_error(PG_ENOMEM, "reciprocal");
         CarryNode c = lin_node(e.d, em);
         c.out_len = el;
@@ -269,12 +327,17 @@ int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, s
         c.ysh = -(int64_t)s;
         c.yneg = 1;
         c.cin = 1;
-        c.flag = flags.d + std::min<size_t>(step, 63);
+        c.flag = flags.d + 1 + std::min<size_t>(step, 62);
         if ((st = lin_one(c, "newton"))) return st;
+        if (checked) {   // two_scale - bx of unsigned precn_t saturates at 0 (sub.cpp:14)
+            uint64_t pos = 0;
+            if ((st = read_words(c.flag, 1, &pos))) return st;
+            if (!pos && (st = dset(e.d, el, nullptr, 0))) return st;
+        }
         if (!xe.reset(xl + el)) return set_error(PG_ENOMEM, "reciprocal");
         if ((st = dmul(xe.d, x.d, xl, e.d, el))) return st;
         // x = (x << s) * e >> next = x * e >> (next - s)
-        const size_t nxl = limbs_for_bits(nk + 2) + 1;
+        const size_t nxl = std::max(limbs_for_bits(nk + 2) + 1, xl);
         if (!xn.reset(nxl)) return set_error(PG_ENOMEM, "reciprocal");
         CarryNode d = lin_node(xn.d, nxl);
         d.x = xe.d;
@@ -286,40 +349,85 @@ int recip_approx(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t nb, s
         known = nk;
         ++step;
     }
-    if (step) {
-        std::vector<uint64_t> f(std::min<size_t>(step, 64));
-        if ((st = read_words(flags.d, f.size(), f.data()))) return st;
-        for (uint64_t v : f)
-            if (v != 1) return set_error(PG_EINTERNAL, "Newton reciprocal overshoot");
+    if (step && !checked) {
+        std::vector<uint64_t> f(std::min<size_t>(step, 63));
+        if ((st = read_words(flags.d + 1, f.size(), f.data()))) return st;
+        for (uint64_t v : f) overshoot |= v != 1;
     }
     return PG_OK;
 }
 
+int div_mulinv_at(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size_t an, const uint64_t *b,
+                  size_t bn, int depth);
+
 // Brings a quotient estimate q (ql limbs) to the floor: r (rm limbs, two's
 // complement, sign word at *flag) is the remainder of q.  The two loops of
-// div_newton.cpp:78-85 / 114-125, one unit per step.
-static int correct(DBuf &q, size_t ql, DBuf &r, size_t rm, uint64_t *flag, const uint64_t *b, size_t bn) {
-    DBuf q2(ql), r2(rm);
+// div_newton.cpp:114-125: while r < 0, q -= delta and r += b delta with
+// delta = max(1, floor(-r / b)); while r >= b, q += delta and r -= b delta
+// with delta = floor(r / b).  A good estimate leaves at most a unit or two,
+// taken as unit steps; past two steps delta comes from a nested division of
+// |r| by b (whose own estimate is within a few units), so any error is
+// corrected in a bounded number of rounds.
+static int correct(DBuf &q, size_t ql, DBuf &r, size_t rm, uint64_t *flag, const uint64_t *b, size_t bn,
+                   int depth = 0) {
+    DBuf q2(ql), r2(rm), m, p;
     if (!q2.d || !r2.d) return set_error(PG_ENOMEM, "division correction");
     int st;
-    for (int it = 0; it < 64; ++it) {
+    for (int it = 0; it < 2 + 64; ++it) {
         uint64_t f = 0;
         if ((st = read_words(flag, 1, &f))) return st;
-        if (f == 0) {   // r < 0: q -= 1, r += b
-            CarryNode c[2] = {lin_node(q2.d, ql), lin_node(r2.d, rm)};
-            c[0].x = q.d; c[0].xn = ql; c[0].yneg = 1;          // q + ~0 = q - 1
-            c[1].x = r.d; c[1].xn = rm; c[1].y = b; c[1].yn = bn; c[1].flag = flag;
-            if ((st = lin_run(c, 2, "correct"))) return st;
-        } else {        // r >= 0: done unless r >= b
+        const bool neg = f == 0;
+        if (!neg) {   // r >= 0: done unless r >= b
             CarryNode d = lin_node(r2.d, rm);
             d.x = r.d; d.xn = rm; d.y = b; d.yn = bn; d.yneg = 1; d.cin = 1; d.flag = flag;
             if ((st = lin_one(d, "correct"))) return st;
             uint64_t g = 0;
             if ((st = read_words(flag, 1, &g))) return st;
             if (g == 0) return PG_OK;
-            CarryNode c = lin_node(q2.d, ql);
-            c.x = q.d; c.xn = ql; c.cin = 1;
+        }
+        if (it < 2) {   // one unit
+            if (neg) {   // q -= 1, r += b
+                CarryNode c[2] = {lin_node(q2.d, ql), lin_node(r2.d, rm)};
+                c[0].x = q.d; c[0].xn = ql; c[0].yneg = 1;          // q + ~0 = q - 1
+                c[1].x = r.d; c[1].xn = rm; c[1].y = b; c[1].yn = bn; c[1].flag = flag;
+                if ((st = lin_run(c, 2, "correct"))) return st;
+            } else {     // q += 1; r2 = r - b and its sign word (>= 0) are already there
+                CarryNode c = lin_node(q2.d, ql);
+                c.x = q.d; c.xn = ql; c.cin = 1;
+                if ((st = lin_one(c, "correct"))) return st;
+            }
+        } else {   // delta = floor(|r| / b), at least 1
+            if (depth >= 3) return set_error(PG_EINTERNAL, "quotient correction did not converge");
+            if (!m.reset(rm)) return set_error(PG_ENOMEM, "division correction");
+            CarryNode c = lin_node(m.d, rm);   // |r|
+            c.x = r.d; c.xn = rm;
+            if (neg) {
+                c.x = nullptr; c.xn = 0; c.y = r.d; c.yn = rm; c.yneg = 1; c.cin = 1;   // -r
+            }
             if ((st = lin_one(c, "correct"))) return st;
+            size_t mn = 0;
+            if ((st = top_len(m.d, rm, &mn))) return st;
+            DBuf dq;
+            size_t dl = 0;
+            if ((st = div_mulinv_at(dq, dl, nullptr, nullptr, m.d, mn, b, bn, depth + 1))) return st;
+            if (!dl) {   // |r| < b: delta = 1
+                const uint64_t one = 1;
+                if (!dq.reset(1)) return set_error(PG_ENOMEM, "division correction");
+                if ((st = dset(dq.d, 1, &one, 1))) return st;
+                dl = 1;
+            }
+            if (!p.reset(bn + dl)) return set_error(PG_ENOMEM, "division correction");
+            if ((st = dmul(p.d, b, bn, dq.d, dl))) return st;
+            CarryNode u[2] = {lin_node(q2.d, ql), lin_node(r2.d, rm)};
+            u[0].x = q.d; u[0].xn = ql; u[0].y = dq.d; u[0].yn = dl;                   // q + delta
+            u[1].x = r.d; u[1].xn = rm; u[1].y = p.d; u[1].yn = bn + dl;               // r - b delta
+            if (neg) {
+                u[0].yneg = 1; u[0].cin = 1;                                          // q - delta
+                u[1].flag = flag;                                                     // r + b delta
+            } else {
+                u[1].yneg = 1; u[1].cin = 1; u[1].flag = flag;
+            }
+            if ((st = lin_run(u, 2, "correct"))) return st;
         }
         std::swap(q, q2);
         std::swap(r, r2);
@@ -355,6 +463,11 @@ int recip_finish(DBuf &x, size_t &xl, const uint64_t *b, size_t bn, size_t bits)
 
 int div_mulinv(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size_t an, const uint64_t *b,
                size_t bn) {
+    return div_mulinv_at(q, ql, r, rl, a, an, b, bn, 0);
+}
+
+int div_mulinv_at(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size_t an, const uint64_t *b,
+                  size_t bn, int depth) {
     ql = 0;
     if (rl) *rl = 0;
     if (!an || !bn) return PG_OK;   // div_newton.cpp:90-93
@@ -382,6 +495,13 @@ int div_mulinv(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size
     if (!q.reset(q0)) return set_error(PG_ENOMEM, "division");
     CarryNode c = lin_node(q.d, q0);
     c.x = ax.d; c.xn = an + xl; c.xsh = (int64_t)scale;
+    if (depth == 0 && g_perturb_q) {   // test hook: q0 +- |amount| units
+        const uint64_t k = (uint64_t)(g_perturb_q < 0 ? -g_perturb_q : g_perturb_q);
+        if ((st = dset(ctx().words + 10, 1, &k, 1))) return st;
+        c.y = ctx().words + 10; c.yn = 1;
+        if (g_perturb_q < 0) { c.yneg = 1; c.cin = 1; }
+        g_perturb_q = 0;
+    }
     if ((st = lin_one(c, "newton"))) return st;
     ax.release();
     DBuf p(bn + q0);
@@ -395,7 +515,7 @@ int div_mulinv(DBuf &q, size_t &ql, DBuf *r, size_t *rl, const uint64_t *a, size
     d.flag = ctx().words + 9;
     if ((st = lin_one(d, "correct"))) return st;
     p.release();
-    if ((st = correct(q, q0, rr, rm, ctx().words + 9, b, bn))) return st;
+    if ((st = correct(q, q0, rr, rm, ctx().words + 9, b, bn, depth))) return st;
     if ((st = top_len(q.d, q0, &ql))) return st;
     if (r) {
         if ((st = top_len(rr.d, std::min(rm, bn), rl))) return st;

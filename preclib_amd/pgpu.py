@@ -26,7 +26,7 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
            "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
            "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_host_staging",
-           "pg_mul_dev_static", "pg_from_decimal"]
+           "pg_mul_dev_static", "pg_from_decimal", "pg_debug_perturb"]
 NTT_VARIANTS = 16
 OP_MUL, OP_DIV = 0, 1
 
@@ -233,6 +233,13 @@ def from_decimal(s):
     L.pg_from_decimal.argtypes = [P64, SZ, ctypes.POINTER(SZ), ctypes.c_char_p, SZ]
     _check(L.pg_from_decimal(_ptr(out), cap, ctypes.byref(n), b, len(b)))
     return out[: n.value].copy()
+
+
+def debug_perturb(what, amount):
+    """pg_debug_perturb: one-shot perturbation of the next division (tests)."""
+    L = load()
+    L.pg_debug_perturb.argtypes = [ctypes.c_int, ctypes.c_int64]
+    _check(L.pg_debug_perturb(what, amount))
 
 
 def host_staging():

@@ -115,3 +115,48 @@ def test_device_resident(pg, oracle):
     da2 = pg.DevBuf.of(a2)
     ql, _ = pg.div_dev(dq, None, da2, len(a2), db, len(b))
     assert np.array_equal(dq.download(ql), eq)
+
+
+@pytest.mark.parametrize("amount", [1000, -1000, 10 ** 6, -(10 ** 6), 1 << 40, 3])
+@pytest.mark.parametrize("shape", [(20000, 7000), (300, 60), (9000, 4500)])
+def test_quotient_correction_from_far(pg, oracle, shape, amount):
+    """The first quotient estimate moved by `amount` units (test hook): the
+    correction of div_newton.cpp:114-125 (delta = floor(|r| / b), not unit
+    steps) still ends at the exact floor."""
+    an, bn = shape
+    a, b = W.pair(an, bn, "uniform", 91 + an)
+    wq, wr = oracle.divmod(a, b)
+    print("case perturb q", shape, amount, flush=True)
+    pg.debug_perturb(0, amount)
+    q, r = pg.divmod_(a, b)
+    assert np.array_equal(q, wq) and np.array_equal(r, wr)
+    q2, r2 = pg.divmod_(a, b)   # one-shot: the next division is unperturbed
+    assert np.array_equal(q2, wq) and np.array_equal(r2, wr)
+
+
+@pytest.mark.parametrize("mul", [2, 3, 6, 1000])
+def test_reciprocal_overshoot_branch(pg, oracle, mul):
+    """A Newton seed `mul` times too large (test hook) takes the halving
+    branch of div_newton.cpp:57-62, with the unsigned difference saturating
+    at zero when halving is not enough; the reciprocal and the quotient are
+    still exact."""
+    b = W.operand(3000, 5)
+    bits = 3000 * 64 + 5000
+    print("case overshoot", mul, flush=True)
+    pg.debug_perturb(1, mul)
+    assert np.array_equal(pg.recip(b, bits), oracle.recip(b, bits))
+    a = W.operand(9000, 6)
+    wq, wr = oracle.divmod(a, b)
+    pg.debug_perturb(1, mul)
+    q, r = pg.divmod_(a, b)
+    assert np.array_equal(q, wq) and np.array_equal(r, wr)
+
+
+def test_large_division_perturbed(pg, golden_div):
+    """pi's shape (2^21 / 2^20 limbs) with the estimate 1000 units low:
+    the GMP digests still match."""
+    c = [c for c in golden_div["div"] if c["an"] == 1 << 21][0]
+    a, b = W.pair(c["an"], c["bn"], "uniform", c["k"])
+    pg.debug_perturb(0, -1000)
+    q, r = pg.divmod_(a, b)
+    assert sha(q) == c["q_sha256"] and sha(r) == c["r_sha256"]
