@@ -1,0 +1,238 @@
+"""Index bounds of the carry kernels (pg_carry.hip: k_carry_spec, k_carry_fix)
+on the CPU.
+
+The kernels' global accesses are replayed index by index for launch shapes
+built as the host builds them (lin_run and Batch::build: a node's chunks
+start at chunk0 = the sum of the chunks before it, and a node covers
+ceil(max(m (+1 with a flag word), out_len) / CHUNK) chunks): every chunk,
+wave, lane and iteration, with the kernel constants (CHUNK = 2048 limbs,
+8 groups of 64 per wave, 4 waves).  Each read and write must fall inside its
+array: the limb sums (m entries, 2 m for half-limb parts), the blocks of a
+blocked node (len entries each), the linear operands (xn / yn), the output
+(out_len), the chunk map and aggregates (nchunks); the fix-up's look-back
+reads only aggregates below its chunk and ends (at the latest at its node's
+first chunk, which the first pass always resolves).
+"""
+import random
+
+import numpy as np
+import pytest
+
+CHUNK, ITERS, WAVES = 2048, 8, 4          # pg_kernels.h, pg_carry.hip
+PLAIN, BLOCKED, LIN = 0, 1, 2
+
+
+class Node:
+    def __init__(self, kind, m, out_len, parts=1, B=0, blocks=(), xn=0, xsh=0, yn=0, ysh=0, flag=False):
+        self.kind, self.m, self.out_len, self.parts = kind, m, out_len, parts
+        self.B, self.blocks = B, list(blocks)
+        self.xn, self.xsh, self.yn, self.ysh, self.flag = xn, xsh, yn, ysh, flag
+        self.chunk0 = 0
+
+
+def ceil_chunks(v):
+    return (v + CHUNK - 1) // CHUNK
+
+
+def chunks_lin(n):       # pg_dint.hip node_chunks (lin_run)
+    return max(1, ceil_chunks(max(n.m + (1 if n.flag else 0), n.out_len)))
+
+
+def chunks_batch(n):     # pg_batch.hip Batch::build, and the single plain carry of pg_runtime.hip
+    return ceil_chunks(max(n.m, n.out_len))
+
+
+def launch(nodes, chunks=chunks_lin):
+    """chunk0 of every node and the chunk map, as the host writes them."""
+    cmap, c = [], 0
+    for i, n in enumerate(nodes):
+        n.chunk0 = c
+        k = chunks(n)
+        cmap += [i] * k
+        c += k
+    return cmap
+
+
+def shifted_reads(n, sh, j):
+    """Indices shifted_limb(v, n, sh, j) reads (pg_carry.hip)."""
+    out = []
+    if sh >= 0:
+        i = j + (sh >> 6)
+        if i < n:
+            out.append(i)
+        if sh & 63 and i + 1 < n:
+            out.append(i + 1)
+    else:
+        q, r = (-sh) >> 6, (-sh) & 63
+        if j >= q and j - q < n:
+            out.append(j - q)
+        if r and j >= q + 1 and j - q - 1 < n:
+            out.append(j - q - 1)
+    return out
+
+
+def limb_sum_reads(N, j):
+    """(array, index) pairs limb_sum(N, j) reads."""
+    if N.kind == LIN:
+        return [("x", i) for i in shifted_reads(N.xn, N.xsh, j)] + [("y", i) for i in shifted_reads(N.yn, N.ysh, j)]
+    if N.kind == PLAIN:
+        if N.parts == 1:
+            return [("sums", j)]
+        return [("sums", 2 * j), ("sums", 2 * j + 1)]
+    out = []
+    k = j // N.B
+    for d in (0, 1):
+        if k < d:
+            break
+        kb = k - d
+        if kb >= len(N.blocks):
+            continue
+        local = j - kb * N.B
+        if local < N.blocks[kb]:
+            out.append(("block%d" % kb, local))
+    return out
+
+
+def check_node_access(N, arr, i):
+    size = {"x": N.xn, "y": N.yn, "sums": N.m * N.parts, "out": N.out_len, "out_hi": N.out_len}.get(arr)
+    if arr.startswith("block"):
+        size = N.blocks[int(arr[5:])]
+    assert 0 <= i < size, (arr, i, size)
+
+
+def t_reads(N, j, plain):
+    """Reads of limbs_t_plain (plain) or limb_t for limb j of the window."""
+    if plain:
+        # limbs_t_plain: indices clamped into [0, m) before the loads
+        if not N.m:
+            return []
+        jl = 0 if j < 0 else (j if j < N.m else N.m - 1)
+        jh = 0 if j < 1 else (j - 1 if j - 1 < N.m else N.m - 1)
+        return [("sums", jl), ("sums", jh)]
+    if j < 0:
+        return []
+    out = []
+    if 1 <= j <= N.m:
+        out += limb_sum_reads(N, j - 1)
+    if j < N.m:
+        out += limb_sum_reads(N, j)
+    return out
+
+
+def writes(N, j):
+    return [("out", j), ("out_hi", j)] if j < N.out_len else []
+
+
+def spec_accesses(nodes, cmap, plain, reads=t_reads):
+    for chunk in range(len(cmap)):
+        assert 0 <= cmap[chunk] < len(nodes)
+        N = nodes[cmap[chunk]]
+        start = chunk == 0 if plain else N.chunk0 == chunk
+        cbase = (chunk - N.chunk0) * CHUNK
+        assert cbase >= 0
+        for wave in range(WAVES):
+            base = cbase + wave * 64 * ITERS + np.arange(64)
+            window = wave == 0 and not start
+            b0 = base - 64 if window else base
+            for it in range(ITERS + (1 if window else 0)):
+                for j in (b0 + 64 * it).tolist():
+                    for a, i in reads(N, j, plain):
+                        check_node_access(N, a, i)
+            for it in range(ITERS):
+                for j in (base + 64 * it).tolist():
+                    for a, i in writes(N, j):
+                        check_node_access(N, a, i)
+
+
+def fix_accesses(nodes, cmap, agg_resolved, plain):
+    """k_carry_fix: the look-back reads agg[i] only for 0 <= i < chunk and
+    ends at the latest at the node's first chunk; the ripple reads and
+    writes the chunk's own limbs."""
+    nch = len(cmap)
+    for blk in range((nch + WAVES - 1) // WAVES):
+        for wave in range(WAVES):
+            chunk = blk * WAVES + wave
+            if chunk >= nch or agg_resolved[chunk]:
+                continue
+            N = nodes[cmap[chunk]]
+            top = chunk - 1
+            while True:
+                idx = [top - lane for lane in range(64)]
+                assert all(i < chunk for i in idx)
+                # read only where i >= 0; lanes below 0 decide (no chunk below)
+                dec = [i < 0 or agg_resolved[i] for i in idx]
+                if any(dec):
+                    first = idx[dec.index(True)]
+                    assert first >= N.chunk0, (chunk, first, N.chunk0)
+                    break
+                top -= 64
+            cbase = (chunk - N.chunk0) * CHUNK
+            for j in range(cbase, cbase + CHUNK):
+                for a, i in t_reads(N, j, plain) + writes(N, j):
+                    check_node_access(N, a, i)
+
+
+def worst_case_resolution(nodes, cmap):
+    """Only node starts resolved: the longest look-backs the fix-up can take."""
+    return [nodes[cmap[c]].chunk0 == c for c in range(len(cmap))]
+
+
+def check_launch(nodes, cmap, plain):
+    spec_accesses(nodes, cmap, plain)
+    fix_accesses(nodes, cmap, worst_case_resolution(nodes, cmap), plain)
+
+
+@pytest.mark.parametrize("m,out_len", [(1, 1), (1, 2), (2047, 2048), (2048, 2048), (2048, 2049), (4096, 4097),
+                                       (5000, 4999), (6143, 6144), (10240, 10241), (3, 9000)])
+def test_single_plain_launch(m, out_len):
+    # the product's carry: one plain node with whole-limb sums, no node table
+    nodes = [Node(PLAIN, m, out_len)]
+    check_launch(nodes, launch(nodes, chunks_batch), plain=True)
+
+
+def random_lin(rng, m):
+    return Node(LIN, m, rng.choice([m, max(1, m - 1)]), xn=rng.randrange(0, m + 3), yn=rng.randrange(0, m + 3),
+                xsh=rng.choice([0, 1, 63, 64, 65, 200, -1, -64, -130]), ysh=rng.choice([0, 5, -5, 128, -128]),
+                flag=rng.random() < 0.5)
+
+
+def random_batch_node(rng, m):
+    if rng.random() < 0.6:
+        return Node(PLAIN, m, m + rng.choice([0, 1, 40]), parts=rng.choice([1, 2]))
+    # mul_unbalanced_into's blocks: B = 2 sn limbs of the longer operand, each
+    # block product pn + sn long (pg_batch.hip, blocked plan)
+    sn = rng.randrange(1, 800)
+    ln = rng.randrange(sn, 6 * sn + 1)
+    B = 2 * sn
+    blocks = [min(B, ln - off) + sn for off in range(0, ln, B)]
+    plen = ln + sn
+    return Node(BLOCKED, plen, plen + rng.choice([0, 0, 1]), B=B, blocks=blocks)
+
+
+SIZES = [1, 2, 63, 64, 65, 2047, 2048, 2049, 4095]
+
+
+def test_linear_launches():
+    rng = random.Random(7)
+    for _ in range(10):
+        nodes = [random_lin(rng, rng.choice(SIZES + [rng.randrange(1, 7000)])) for _ in range(rng.randrange(1, 5))]
+        check_launch(nodes, launch(nodes, chunks_lin), plain=False)
+
+
+def test_batch_levels():
+    rng = random.Random(11)
+    for _ in range(10):
+        nodes = [random_batch_node(rng, rng.choice(SIZES + [rng.randrange(1, 7000)]))
+                 for _ in range(rng.randrange(1, 5))]
+        check_launch(nodes, launch(nodes, chunks_batch), plain=False)
+
+
+def test_model_catches_an_unclamped_window():
+    """The bound check has teeth: limb loads without the clamp of
+    limbs_t_plain (a plausible slip) read sums[-1] at limb 0 and sums[m] and
+    beyond in the chunk that holds only the top word and out_len."""
+    def unclamped(N, j, plain):
+        return [("sums", j), ("sums", j - 1)]
+    nodes = [Node(PLAIN, 4096, 4097)]
+    with pytest.raises(AssertionError):
+        spec_accesses(nodes, launch(nodes, chunks_batch), True, reads=unclamped)

@@ -117,3 +117,40 @@ def test_staging_images_recycled(pg):
         held, spare = pg.host_staging()
         assert held == 0, (k, held)
         assert spare <= cap, (k, spare)
+
+
+def test_uneven_batch_4096_graph_replays(pg, golden):
+    """The whole 4096-product batch as a planned device batch
+    (pg_batch_plan / pg_batch_run): captured into a hipGraph on its first
+    run and replayed twice, with the outputs clobbered and eager work in
+    between; every run matches the golden digest of digests."""
+    shapes = W.batch_shapes()
+    trip, bufs = [], []
+    for k, (an, bn) in enumerate(shapes):
+        a, b = W.pair(an, bn, "uniform", k)
+        da, db, dr = pg.DevBuf.of(a), pg.DevBuf.of(b), pg.DevBuf(an + bn)
+        bufs += [da, db, dr]
+        trip.append((dr, da, an, db, bn))
+    batch = pg.DevBatch(trip)
+    try:
+        for rep in range(3):
+            for (dr, _, an, _, bn) in trip:
+                dr.upload(np.full(an + bn, 0xCD, np.uint64))
+            print("case batch run", rep, flush=True)
+            batch.run()
+            pg.sync()
+            per = []
+            for (dr, _, an, _, bn) in trip:
+                x = dr.download(an + bn)
+                n = len(x)
+                while n and x[n - 1] == 0:
+                    n -= 1
+                per.append(digest(x[:n]))
+            assert per[:64] == golden["batch"]["first"], rep
+            assert hashlib.sha256("".join(per).encode()).hexdigest() == golden["batch"]["digest_of_digests"], rep
+            x, y = W.pair(5000, 3000, "uniform", k=rep)   # eager work between replays
+            assert np.array_equal(pg.mul(x, y), pg.mul(x, y, path=pg.PATH_NTT))
+    finally:
+        batch.free()
+        for d in bufs:
+            d.free()
