@@ -1,6 +1,7 @@
 // pg_batch.h -- many products planned and launched together (pg_batch.hip).
 #pragma once
 #include <map>
+#include <tuple>
 #include <utility>
 #include <vector>
 #include "pg_runtime.h"
@@ -31,6 +32,10 @@ private:
         size_t spec = 0, prod = 0, maxerr = 0;
         bool has_spec = false;
         int parts = 1, job = -1;
+        // operand k's spectra from leaf ext_leaf[k] (its operand ext_op[k]) of
+        // the same bucket; -1: transformed by this leaf
+        long ext_leaf[2] = {-1, -1};
+        int ext_op[2] = {0, 0};
     };
     struct SmallPlan {
         const uint64_t *x, *y;
@@ -70,6 +75,9 @@ private:
     char *arena = nullptr;
     size_t njobs = 0;
     std::map<std::pair<int, int>, std::vector<LeafPlan>> ntt, fft;   // (mode|bits, logn)
+    // first NTT leaf (index in its bucket, operand) transforming each
+    // (mode, logn, operand pointer, length)
+    std::map<std::tuple<int, int, const uint64_t *, size_t>, std::pair<long, int>> spec_owner;
     std::vector<SmallPlan> small;
     std::vector<NodePlan> nodes;
     std::vector<Bucket> buckets;

@@ -124,13 +124,35 @@ int Batch::plan(const uint64_t *x, size_t xn, const uint64_t *y, size_t yn, int 
         if (logn > NTT_MAX_LOG) return set_error(PG_ERANGE, "NTT leaf too long");
         const size_t n = (size_t)1 << logn;
         const int np = mode_primes(mode);
+        auto &bucket = ntt[{mode, logn}];
         LeafPlan lp{};
         lp.x = x; lp.y = y; lp.xn = xn; lp.yn = yn;
         lp.sums = leaf_target(plen);
-        lp.spec = plan_l1(logn) ? alloc->take(2 * np * n * 4) : 0;
-        lp.prod = alloc->take(np * n * 4);
         lp.has_spec = plan_l1(logn) > 0;
-        ntt[{mode, logn}].push_back(lp);
+        if (lp.has_spec) {
+            // An operand already transformed in this bucket (the short operand
+            // of every block of a blocked multiply, a factor shared by two
+            // products of a level, both operands of a square) is transformed
+            // once: the forward column kernel of the bucket's one launch
+            // finishes before its row kernel reads the spectra.
+            const uint64_t *ptr[2] = {x, y};
+            const size_t len[2] = {xn, yn};
+            for (int op = 0; op < 2; ++op) {
+                const auto key = std::make_tuple(mode, logn, ptr[op], len[op]);
+                const auto it = spec_owner.find(key);
+                if (it != spec_owner.end()) {
+                    lp.ext_leaf[op] = it->second.first;
+                    lp.ext_op[op] = it->second.second;
+                } else {
+                    spec_owner[key] = {(long)bucket.size(), op};
+                }
+            }
+            // spectra this leaf writes: [operand][prime][n] up to its last own operand
+            const int nown = lp.ext_leaf[1] < 0 ? 2 : lp.ext_leaf[0] < 0 ? 1 : 0;
+            lp.spec = alloc->take((size_t)nown * np * n * 4);
+        }
+        lp.prod = alloc->take(np * n * 4);
+        bucket.push_back(lp);
         return PG_OK;
     }
     case 3: {
@@ -199,6 +221,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
     Alloc al;
     alloc = &al;
     njobs = n;
+    spec_owner.clear();
     for (size_t i = 0; i < n; ++i) {
         const pg_job &j = jobs[i];
         if (!j.a || !j.b || !j.r || !j.an || !j.bn || j.r_cap < j.an + j.bn)
@@ -250,10 +273,14 @@ int Batch::build(const pg_job *jobs, size_t n) {
         b.logn = kv.first.second;
         b.first = k;
         b.count = kv.second.size();
+        const size_t pn = (size_t)mode_primes(b.mode) << b.logn;   // spectra of one operand
         for (auto &lp : kv.second) {
             NttLeaf L{};
             L.a = lp.x; L.b = lp.y; L.an = lp.xn; L.bn = lp.yn;
             L.spec = lp.has_spec ? (uint32_t *)P(lp.spec) : nullptr;
+            for (int op = 0; op < 2; ++op)
+                if (lp.ext_leaf[op] >= 0)
+                    L.ext[op] = (const uint32_t *)P(kv.second[lp.ext_leaf[op]].spec) + lp.ext_op[op] * pn;
             L.prod = (uint32_t *)P(lp.prod);
             L.slo = SLO(lp.sums); L.shi = SHI(lp.sums);
             L.sum_len = lp.xn + lp.yn;

@@ -152,6 +152,12 @@ struct NttLeaf {
     const uint64_t *a, *b;      // operand limbs (device)
     size_t an, bn;              // limb counts
     uint32_t *spec;             // forward spectra [2 operands][nprimes][n]
+    // Operand k's forward column spectra [nprimes][n] when they come from
+    // elsewhere (null: computed into spec by this leaf): the same operand
+    // (pointer and length) transformed by another leaf of the same launch, or
+    // by this leaf's other operand (a square).  The forward column kernel
+    // skips such an operand; the row kernel reads its rows from here.
+    const uint32_t *ext[2];
     uint32_t *prod;             // product spectra / residues [nprimes][n]
     uint64_t *slo, *shi;        // 128-bit limb sums
     size_t sum_len;             // sums written: limbs [0, sum_len)

@@ -98,7 +98,10 @@ extern "C" int pg_mul_dev_static(uint64_t *r, const uint64_t *a, size_t an, cons
         p += up256(bytes);
         return q;
     };
-    if (plan_l1(L.logn, L.variant)) L.one.spec = (uint32_t *)take(2 * 3 * n * 4);
+    if (plan_l1(L.logn, L.variant)) {
+        L.one.spec = (uint32_t *)take(2 * 3 * n * 4);
+        if (a == b && an == bn) L.one.ext[1] = L.one.spec;   // a square: one forward transform
+    }
     L.one.prod = (uint32_t *)take(3 * n * 4);
     L.one.slo = (uint64_t *)take(m * 8);
     L.one.shi = (uint64_t *)take(m * 8);

@@ -486,6 +486,13 @@ __device__ __forceinline__ void put_digits(uint32_t *lds, int i, uint64_t v, uin
     }
 }
 
+// Forward column spectra of operand op, prime pr, of a leaf (see NttLeaf::ext).
+template <int LOGN, int NP>
+__device__ __forceinline__ const uint32_t *op_spec(const NttLeaf &D, int op, int pr) {
+    const uint32_t *x = op ? D.ext[1] : D.ext[0];   // (a select: an indexed read would put D in scratch)
+    return x ? x + ((size_t)pr << LOGN) : D.spec + ((size_t)(op * NP + pr) << LOGN);
+}
+
 // ---- kernel 1: forward column pass ------------------------------------------
 
 // Column transforms of a staged tile (half: the upper half of every column
@@ -534,6 +541,7 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
         pr = (L.order & 3) == COLF_TILE_OP ? blockIdx.y : NP - 1 - blockIdx.y;
     }
     }
+    if (op ? D.ext[1] : D.ext[0]) return;   // transformed by another leaf (or operand) of this launch
     const uint32_t c0 = (uint32_t)tile << LT;
     const uint64_t *src = op ? D.b : D.a;
     const size_t nl = op ? D.bn : D.an;
@@ -606,8 +614,7 @@ __device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, con
     } else if constexpr (NOPS == 1 && (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS > 4) {
         // the longest single rows: one load at a time (that kernel has no
         // registers to spare)
-        const uint4 *in = (const uint4 *)(D.spec + ((size_t)(op0 * mode_primes(MODE) + pr) << LOGN) +
-                                          ((size_t)row << L2));
+        const uint4 *in = (const uint4 *)(op_spec<LOGN, mode_primes(MODE)>(D, op0, pr) + ((size_t)row << L2));
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
             const uint4 v = in[m];
             lds[lp(4 * m)] = v.x; lds[lp(4 * m + 1)] = v.y; lds[lp(4 * m + 2)] = v.z; lds[lp(4 * m + 3)] = v.w;
@@ -622,7 +629,7 @@ __device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, con
             u32x4 v[NOPS][GR];
 #pragma unroll
             for (int k = 0; k < NOPS; ++k) {
-                const auto in = gptr((const u32x4 *)(D.spec + ((size_t)((op0 + k) * mode_primes(MODE) + pr) << LOGN) +
+                const auto in = gptr((const u32x4 *)(op_spec<LOGN, mode_primes(MODE)>(D, op0 + k, pr) +
                                                      ((size_t)row << L2)));
 #pragma unroll
                 for (int i = 0; i < GR; ++i) {
@@ -704,7 +711,7 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
     load_rows<LOGN, MODE, V, 1>(lds, L, D, 0, pr, row);
     // operand B's spectrum row is fetched now and lands during A's transform
     uint4 pb[PF ? E4 : 1];
-    const uint4 *inb = (const uint4 *)(D.spec + ((size_t)(mode_primes(MODE) + pr) << LOGN) + ((size_t)row << L2));
+    const uint4 *inb = (const uint4 *)(op_spec<LOGN, mode_primes(MODE)>(D, 1, pr) + ((size_t)row << L2));
     if constexpr (PF) {
 #pragma unroll
         for (int k = 0; k < E4; ++k) {

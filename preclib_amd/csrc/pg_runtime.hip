@@ -633,7 +633,11 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     const int NP = mode_primes(mode), DPL = 64 / mode_digit_bits(mode);
     Scratch s;
     const int l1 = plan_l1(L.logn, mode == MODE_WIDE ? L.variant : 0);
-    if (l1) L.one.spec = s.get<uint32_t>(2 * NP * n);
+    // a square transforms its operand once: the row kernel reads operand b's
+    // spectra from operand a's
+    const bool sqr = ap == bp && an == bn;
+    if (l1) L.one.spec = s.get<uint32_t>((sqr ? 1 : 2) * NP * n);
+    if (l1 && sqr) L.one.ext[1] = L.one.spec;
     L.one.prod = s.get<uint32_t>(NP * n);
     const size_t m = n / DPL;
     L.one.slo = s.get<uint64_t>(m);

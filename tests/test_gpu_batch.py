@@ -101,6 +101,47 @@ def test_device_batch_graph_replays(pg, oracle):
     batch.free()
 
 
+def test_shared_operand_spectra(pg, oracle):
+    """Products of one device batch that share an operand (same buffer and
+    length, same transform length) transform it once: a in both operand
+    slots, a square, and the short operand of a blocked product whose last
+    block lands in the same transform length.  Every product, on the eager
+    run and on two graph replays, equals the oracle's."""
+    a, b = W.pair(3000, 3000, "uniform", k=410)
+    c, big = W.pair(2900, 9000, "uniform", k=411)
+    da, db, dc, dbig = pg.DevBuf.of(a), pg.DevBuf.of(b), pg.DevBuf.of(c), pg.DevBuf.of(big)
+    ops = {"a": (da, a), "b": (db, b), "c": (dc, c), "big": (dbig, big)}
+    pairs = [("a", "b"), ("a", "c"), ("c", "a"), ("a", "a"), ("b", "a"), ("big", "a"), ("c", "c")]
+    trip, want, outs = [], [], []
+    for x, y in pairs:
+        (dx, hx), (dy, hy) = ops[x], ops[y]
+        dr = pg.DevBuf(len(hx) + len(hy))
+        outs.append(dr)
+        trip.append((dr, dx, len(hx), dy, len(hy)))
+        want.append(oracle.mul_into(hx, hy))
+    batch = pg.DevBatch(trip)
+    for rep in range(3):
+        for dr, _, an, _, bn in trip:
+            dr.upload(np.full(an + bn, 0xCD, np.uint64))
+        batch.run()
+        pg.sync()
+        for (dr, _, an, _, bn), w, pq in zip(trip, want, pairs):
+            got = dr.download(an + bn)
+            assert np.array_equal(got[: len(w)], w) and not got[len(w):].any(), (rep, pq)
+    batch.free()
+
+
+def test_square_transforms_once(pg, oracle):
+    """A square through pg_mul_dev (one buffer in both operand slots) reads
+    its second operand's spectra from the first's, at sizes on every plan
+    geometry class."""
+    for n in [200, 3000, 40000, 300000]:
+        a = W.operand(n, 4200 + n)
+        da, dr = pg.DevBuf.of(a), pg.DevBuf(2 * n)
+        rl, _ = pg.mul_dev(dr, da, n, da, n)
+        assert np.array_equal(dr.download(rl), oracle.mul_into(a, a)), n
+
+
 def test_staging_images_recycled(pg):
     """Small and unbalanced products upload page-locked descriptor images;
     every synchronous call recycles them, so a long run of pg_mul calls
