@@ -53,6 +53,26 @@ __device__ __forceinline__ uint64_t shifted_limb(const uint64_t *vp, size_t n, i
 
 __device__ __forceinline__ bool lin_cond(const CarryNode &N) { return !N.cond || *gptr(N.cond) == N.cond_eq; }
 
+// Limb sum of a blocked node at block index k and offset r (limb k B + r):
+// the block products k and k - 1 that cover it.
+__device__ __forceinline__ void blocked_sum(const CarryNode &N, size_t k, size_t r, uint64_t &lo, uint64_t &hi) {
+    lo = 0;
+    hi = 0;
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        if (k < (size_t)d) break;
+        const size_t kb = k - d;
+        if (kb >= (size_t)N.nblocks) continue;
+        const auto blk = gptr(N.blocks) + kb;
+        const size_t local = r + d * N.B;
+        if (local < blk->len) {
+            const uint64_t s = lo + gptr(blk->slo)[local];
+            hi += gptr(blk->shi)[local] + (s < lo);
+            lo = s;
+        }
+    }
+}
+
 // Limb sum j of a node as (lo, hi).
 __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t &lo, uint64_t &hi) {
     if (N.kind == NODE_LIN) {
@@ -85,21 +105,8 @@ __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t 
         }
         return;
     }
-    lo = 0;
-    hi = 0;
     const size_t k = j / N.B;
-    for (int d = 0; d < 2; ++d) {
-        if (k < (size_t)d) break;
-        const size_t kb = k - d;
-        if (kb >= (size_t)N.nblocks) continue;
-        const auto blk = gptr(N.blocks) + kb;
-        const size_t local = j - kb * N.B;
-        if (local < blk->len) {
-            const uint64_t s = lo + gptr(blk->slo)[local];
-            hi += gptr(blk->shi)[local] + (s < lo);
-            lo = s;
-        }
-    }
+    blocked_sum(N, k, j - k * N.B, lo, hi);
 }
 
 __device__ __forceinline__ uint64_t limb_t(const CarryNode &N, size_t j, uint32_t &g, uint32_t &p) {
@@ -196,11 +203,50 @@ __device__ __forceinline__ void limbs_t_plain(const CarryNode &N, long base, uin
     }
 }
 
+// Limbs t_j, j = base + 64 it (base >= 0), of a blocked node: the block
+// index j / B and offset j mod B are stepped from one limb to the next (B is
+// at least 50 limbs) instead of divided per limb.
+template <int IT>
+__device__ __forceinline__ void limbs_t_blocked(const CarryNode &N, long base, uint64_t (&t)[IT],
+                                                uint32_t (&g)[IT], uint32_t (&p)[IT]) {
+    const size_t B = N.B, m = N.m;
+    size_t k = (size_t)base / B, r = (size_t)base - k * B;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const size_t j = (size_t)base + 64 * it;
+        uint64_t lo, hi, hprev = 0;
+        if (j && j <= m) {   // top word of limb j - 1's sum
+            const bool wrap = r == 0;
+            blocked_sum(N, wrap ? k - 1 : k, wrap ? B - 1 : r - 1, lo, hprev);
+        }
+        if (j < m) {
+            blocked_sum(N, k, r, lo, hi);
+            t[it] = lo + hprev;
+            g[it] = t[it] < lo;
+            p[it] = t[it] == ~0ull;
+        } else {
+            t[it] = j == m ? hprev : 0;
+            g[it] = 0;
+            p[it] = 0;
+        }
+        r += 64;
+        while (r >= B) {
+            r -= B;
+            ++k;
+        }
+    }
+}
+
+// Limbs t_j, j = base + 64 it, of node N.  Launches with a node table take
+// the product form for plain whole-limb nodes (loads issued together) and the
+// stepped form for blocked ones; the node is uniform across the workgroup.
 template <bool PLAIN, int IT>
 __device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint64_t (&t)[IT], uint32_t (&g)[IT],
                                             uint32_t (&p)[IT]) {
-    if constexpr (PLAIN) {
+    if (PLAIN || (N.kind == NODE_PLAIN && N.parts == 1)) {
         limbs_t_plain<IT>(N, base, t, g, p);
+    } else if (N.kind == NODE_BLOCKED && base >= 0) {
+        limbs_t_blocked<IT>(N, base, t, g, p);
     } else {
 #pragma unroll
         for (int it = 0; it < IT; ++it) {

@@ -11,7 +11,10 @@ array: the limb sums (m entries, 2 m for half-limb parts), the blocks of a
 blocked node (len entries each), the linear operands (xn / yn), the output
 (out_len), the chunk map and aggregates (nchunks); the fix-up's look-back
 reads only aggregates below its chunk and ends (at the latest at its node's
-first chunk, which the first pass always resolves).
+first chunk, which the first pass always resolves).  Node-table launches
+read plain whole-limb nodes with the product form's clamped loads, and
+blocked nodes with a block index and offset stepped from limb to limb; the
+stepping is checked against division at every limb it visits.
 """
 import random
 
@@ -100,9 +103,25 @@ def check_node_access(N, arr, i):
     assert 0 <= i < size, (arr, i, size)
 
 
+def blocked_steps(N, base, iters):
+    """(k, r) of limbs base + 64 it as limbs_t_blocked steps them (one
+    division, then r += 64 with carries into k); each must equal divmod."""
+    k, r = divmod(base, N.B)
+    out = []
+    for it in range(iters):
+        assert (k, r) == divmod(base + 64 * it, N.B)
+        out.append((k, r))
+        r += 64
+        while r >= N.B:
+            r -= N.B
+            k += 1
+    return out
+
+
 def t_reads(N, j, plain):
-    """Reads of limbs_t_plain (plain) or limb_t for limb j of the window."""
-    if plain:
+    """Reads of limbs_t_plain (the product launch, and plain whole-limb nodes
+    of node-table launches) or limb_t for limb j of the window."""
+    if plain or (N.kind == PLAIN and N.parts == 1):
         # limbs_t_plain: indices clamped into [0, m) before the loads
         if not N.m:
             return []
@@ -134,6 +153,9 @@ def spec_accesses(nodes, cmap, plain, reads=t_reads):
             base = cbase + wave * 64 * ITERS + np.arange(64)
             window = wave == 0 and not start
             b0 = base - 64 if window else base
+            if N.kind == BLOCKED and not plain:
+                for lane_base in b0.tolist():
+                    blocked_steps(N, lane_base, ITERS + (1 if window else 0))
             for it in range(ITERS + (1 if window else 0)):
                 for j in (b0 + 64 * it).tolist():
                     for a, i in reads(N, j, plain):
