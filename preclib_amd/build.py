@@ -50,6 +50,7 @@ def build(verbose=False, out=LIB, defines=()):
 
 HOST_LIB = os.path.join(HERE, "libprecn.so")
 CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_precn")
+CPP_BRANCH = os.path.join(ROOT, "tests", "cpp", "integration_branch")
 CXX = os.environ.get("CXX", "g++")
 
 
@@ -64,6 +65,10 @@ def build_host(verbose=False):
     _run([CXX, "-O2", "-std=c++17", "-I" + inc, "-I" + orc, os.path.join(ROOT, "tests", "cpp", "test_precn.cpp"),
           "-L" + HERE, "-lprecn", "-lpgpu", "-L" + orc, "-loracle",
           "-Wl,-rpath," + HERE + ":" + orc + ":$ORIGIN/../../preclib_amd:$ORIGIN/../../oracle", "-o", CPP_TEST],
+         verbose)
+    # the dispatcher branch INTEGRATION.md shows for preclib's mul_into
+    _run([CXX, "-O2", "-std=c++17", "-I" + inc, os.path.join(ROOT, "tests", "cpp", "integration_branch.cpp"),
+          "-L" + HERE, "-lprecn", "-lpgpu", "-Wl,-rpath," + HERE + ":$ORIGIN/../../preclib_amd", "-o", CPP_BRANCH],
          verbose)
     return HOST_LIB
 
