@@ -85,6 +85,17 @@ constexpr int plan_lt(int logn, int v = 0) {
 }
 // Radix bits of the variant's register passes.
 constexpr int plan_rb(int logn, int v = 0) { return (v >= NTT_GEOMS && logn >= NTT_VARIANT_MIN_LOG) ? 3 : RADIX_BITS; }
+// Forward spectra are stored in groups of 2^plan_gl rows: a group's T columns
+// of one tile are one block of 2^plan_gl x T words, 64 contiguous bytes when
+// 2^plan_gl T = 16 (spec_off in pg_ntt_impl.h).  The forward column kernel
+// then writes 64-byte pieces instead of T-word pieces of separate rows, and
+// the row kernel reads its row in 64-byte pieces.  At 2^24 points (tiles of 8
+// columns) pairs of rows measured 280 -> 249 us forward columns and 320 ->
+// 334 us rows; groups of 4 rows (whole 128-byte lines) 243 and 352 us.
+constexpr int SPEC_GROUP_LOG_WORDS = 4;
+constexpr int plan_gl(int logn, int v = 0) {
+    return cmin(plan_l1(logn, v), cmax(0, SPEC_GROUP_LOG_WORDS - plan_lt(logn, v)));
+}
 // Whether variant v differs from the default at this size.
 constexpr bool plan_distinct(int logn, int v) {
     return v == 0 || plan_l2(logn, v) != plan_l2(logn) || plan_lt(logn, v) != plan_lt(logn) ||

@@ -486,6 +486,16 @@ __device__ __forceinline__ void put_digits(uint32_t *lds, int i, uint64_t v, uin
     }
 }
 
+// Word offset of row j1, column c in a forward spectrum: rows in groups of
+// G = 2^plan_gl, and within a group each tile of T columns stored as a G x T
+// block (plan_gl = 0: row-major).
+template <int LOGN, int V>
+__device__ __forceinline__ size_t spec_off(uint32_t j1, uint32_t c) {
+    constexpr int L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), GL = plan_gl(LOGN, V);
+    return ((size_t)(j1 >> GL) << (L2 + GL)) + ((size_t)(c >> LT) << (LT + GL)) +
+           ((j1 & ((1u << GL) - 1)) << LT) + (c & ((1u << LT) - 1));
+}
+
 // Forward column spectra of operand op, prime pr, of a leaf (see NttLeaf::ext).
 template <int LOGN, int NP>
 __device__ __forceinline__ const uint32_t *op_spec(const NttLeaf &D, int op, int pr) {
@@ -576,7 +586,7 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
 #pragma unroll 4
             for (int k = 0; k < TW::E; ++k) {
                 const int e = threadIdx.x + k * NTT_THREADS;
-                out[((size_t)(e >> LT) << L2) + C] = mont_mul(lds[lp(e)], w, c.p, c.ninv);
+                out[spec_off<LOGN, V>(e >> LT, C)] = mont_mul(lds[lp(e)], w, c.p, c.ninv);
                 w = mont_mul(w, st, c.p, c.ninv);
             }
         } else {
@@ -585,7 +595,7 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
                 const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, RBV>(rho)) & NMASK;
                 uint32_t x = lds[lp(e)];
                 if (ex) x = mont_mul(x, twiddle_mont(L.glo[pr], L.ghi[pr], ex, L.lobits, c), c.p, c.ninv);
-                out[((size_t)rho << L2) + c0 + cc] = x;
+                out[spec_off<LOGN, V>(rho, c0 + cc)] = x;
             }
         }
     }
@@ -600,6 +610,7 @@ template <int LOGN, int MODE, int V, int NOPS>
 __device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, int op0, int pr,
                                           int row) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), N2 = 1 << L2, ROW_THREADS = row_threads(LOGN, V);
+    static_assert(L1 == 0 || plan_lt(LOGN, V) >= 2, "16-byte row loads need tiles of 4 or more columns");
     if constexpr (L1 == 0) {
         // single-row plan: the row is the digit vector itself
         constexpr int DPL = 64 / mode_digit_bits(MODE);
@@ -614,9 +625,9 @@ __device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, con
     } else if constexpr (NOPS == 1 && (N2 / 4 + ROW_THREADS - 1) / ROW_THREADS > 4) {
         // the longest single rows: one load at a time (that kernel has no
         // registers to spare)
-        const uint4 *in = (const uint4 *)(op_spec<LOGN, mode_primes(MODE)>(D, op0, pr) + ((size_t)row << L2));
+        const uint4 *in = (const uint4 *)op_spec<LOGN, mode_primes(MODE)>(D, op0, pr);
         for (int m = threadIdx.x; m < N2 / 4; m += ROW_THREADS) {
-            const uint4 v = in[m];
+            const uint4 v = in[spec_off<LOGN, V>(row, 4 * m) >> 2];
             lds[lp(4 * m)] = v.x; lds[lp(4 * m + 1)] = v.y; lds[lp(4 * m + 2)] = v.z; lds[lp(4 * m + 3)] = v.w;
         }
     } else {
@@ -629,12 +640,11 @@ __device__ __forceinline__ void load_rows(uint32_t *lds, const NttLaunch &L, con
             u32x4 v[NOPS][GR];
 #pragma unroll
             for (int k = 0; k < NOPS; ++k) {
-                const auto in = gptr((const u32x4 *)(op_spec<LOGN, mode_primes(MODE)>(D, op0 + k, pr) +
-                                                     ((size_t)row << L2)));
+                const auto in = gptr((const u32x4 *)op_spec<LOGN, mode_primes(MODE)>(D, op0 + k, pr));
 #pragma unroll
                 for (int i = 0; i < GR; ++i) {
                     const int m = threadIdx.x + (i0 + i) * ROW_THREADS;
-                    if (m < N2 / 4) v[k][i] = in[m];
+                    if (m < N2 / 4) v[k][i] = in[spec_off<LOGN, V>(row, 4 * m) >> 2];
                 }
             }
 #pragma unroll
@@ -711,12 +721,12 @@ __device__ __forceinline__ void row_body(const NttLaunch &L, const NttLeaf &D, i
     load_rows<LOGN, MODE, V, 1>(lds, L, D, 0, pr, row);
     // operand B's spectrum row is fetched now and lands during A's transform
     uint4 pb[PF ? E4 : 1];
-    const uint4 *inb = (const uint4 *)(op_spec<LOGN, mode_primes(MODE)>(D, 1, pr) + ((size_t)row << L2));
+    const uint4 *inb = (const uint4 *)op_spec<LOGN, mode_primes(MODE)>(D, 1, pr);
     if constexpr (PF) {
 #pragma unroll
         for (int k = 0; k < E4; ++k) {
             const int m = threadIdx.x + k * ROW_THREADS;
-            if (m < N2 / 4) pb[k] = inb[m];
+            if (m < N2 / 4) pb[k] = inb[spec_off<LOGN, V>(row, 4 * m) >> 2];
         }
     }
     lds_sync();
@@ -776,7 +786,9 @@ template <int LOGN, int MODE, int V>
 __global__ void __launch_bounds__(row_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(row_waves(LOGN, V))))
 k_row(const NttLaunch L) {
     const NttLeaf D = leaf_of(L);
-    const int row = blockIdx.x, pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
+    // consecutive rows on one XCD: the rows of a spectrum group share its lines
+    const int row = xcd_tile(blockIdx.x, gridDim.x);
+    const int pr = (L.order & ROW_PRIME_REV) ? (int)gridDim.y - 1 - (int)blockIdx.y : blockIdx.y;
     if (pr == LAZY_PRIME) row_body<LOGN, MODE, V, true>(L, D, pr, row);
     else row_body<LOGN, MODE, V, false>(L, D, pr, row);
 }
