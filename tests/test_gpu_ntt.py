@@ -122,9 +122,9 @@ def test_auto_dispatch_squares_and_mid_sizes(pg, oracle, n):
     assert serr == 0.0 and np.array_equal(s, oracle.mul_into(a, a, sqr=True))
 
 
-@pytest.mark.parametrize("logn", list(range(12, 25)))
+@pytest.mark.parametrize("logn", list(range(12, 26)))
 def test_shipped_plan_every_length(pg, oracle, logn):
-    """One product per wide-mode transform length 2^12 .. 2^24 under the
+    """One product per wide-mode transform length 2^12 .. 2^25 under the
     shipped plan table (ntt_plans.txt: non-default variants at several of
     these lengths): equal to the reference-decomposition product
     (PG_PATH_NTT_REF, other kernels and digits), to the oracle up to 2^20
