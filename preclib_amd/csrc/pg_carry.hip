@@ -171,9 +171,56 @@ __device__ __forceinline__ uint32_t node_cin(const CarryLaunch &C, size_t i) {
 // Limbs t_j, j = base + 64 it, of a single plain node with whole-limb sums
 // (the product path).  Every load is issued before the first is used: the
 // indices are clamped into [0, m) and the out-of-range words discarded after.
+// Packed sums (NttLeaf::pack_log): slo[j] is t_j itself except at the first
+// limb of a piece, which adds the high word of the piece before; the carry
+// bits of a piece's other limbs sit above that high word.
+template <int IT>
+__device__ __forceinline__ void limbs_t_packed(const CarryNode &N, long base, uint64_t (&t)[IT],
+                                               uint32_t (&g)[IT], uint32_t (&p)[IT]) {
+    const auto tv = gptr(N.slo), aux = gptr(N.shi);
+    const long m = (long)N.m;
+    const int pl = N.pack_log;
+    constexpr uint64_t HI = (1ull << 57) - 1;   // high words are below 2^57; carry bits are 57 and up
+    uint64_t x[IT], a[IT];
+    if (m) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const long j = base + 64 * it;
+            const long jl = j < 0 ? 0 : (j < m ? j : m - 1);
+            const long pc = jl >> pl, q = jl & ((1l << pl) - 1);
+            x[it] = tv[jl];
+            a[it] = aux[q || pc == 0 ? pc : pc - 1];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const long j = base + 64 * it;
+        const long q = j & ((1l << pl) - 1);
+        if (j >= 0 && j < m) {
+            if (q) {
+                t[it] = x[it];
+                g[it] = (uint32_t)((a[it] >> (56 + q)) & 1);
+            } else {
+                const uint64_t h = j ? (a[it] & HI) : 0;
+                t[it] = x[it] + h;
+                g[it] = t[it] < x[it];
+            }
+            p[it] = t[it] == ~0ull;
+        } else {
+            t[it] = (j >= 0 && j == m) ? (a[it] & HI) : 0;   // top word: the last high word
+            g[it] = 0;
+            p[it] = 0;
+        }
+    }
+}
+
 template <int IT>
 __device__ __forceinline__ void limbs_t_plain(const CarryNode &N, long base, uint64_t (&t)[IT],
                                               uint32_t (&g)[IT], uint32_t (&p)[IT]) {
+    if (N.pack_log) {
+        limbs_t_packed<IT>(N, base, t, g, p);
+        return;
+    }
     const auto slo = gptr(N.slo), shi = gptr(N.shi);
     const long m = (long)N.m;
     uint64_t lo[IT], hp[IT];

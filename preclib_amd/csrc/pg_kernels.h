@@ -172,6 +172,14 @@ struct NttLeaf {
     uint32_t *prod;             // product spectra / residues [nprimes][n]
     uint64_t *slo, *shi;        // 128-bit limb sums
     size_t sum_len;             // sums written: limbs [0, sum_len)
+    // Wide mode, 2^pack_log limbs per row piece of a column tile (0: plain
+    // sums).  Packed, limb j's sum S_j = lo_j + 2^64 hi_j is written as
+    // slo[j] = lo_j + hi_(j-1) mod 2^64 when j is not the first limb of its
+    // piece (slo[j] = lo_j when it is), and shi[j >> pack_log] = hi of the
+    // piece's last limb (< 2^57) with bit 56 + q holding the carry out of
+    // limb q's addition (q = 1 .. 2^pack_log - 1): the carry then reads
+    // about 10 bytes per limb instead of 16.
+    int pack_log;
 };
 
 // A launch covers one leaf (`one`) or, when `leaves` is set, nleaves leaves of
@@ -309,6 +317,7 @@ struct CarryNode {
     uint64_t *out, *out_hi;
     size_t out_len;
     size_t chunk0;              // first chunk of this node in the launch
+    int pack_log;               // plain, whole-limb: sums packed as NttLeaf::pack_log describes
     // NODE_LIN
     const uint64_t *x, *y;
     size_t xn, yn;

@@ -871,6 +871,14 @@ __device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
 }
 
+// Value of lane (this lane - 2) within each row of 16 lanes (DPP row_shr:2;
+// the first two lanes of a row read 0).
+__device__ __forceinline__ uint64_t lane_down2(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x112, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x112, 0xF, 0xF, false);
+    return (uint64_t)hi << 32 | lo;
+}
+
 // Wide mode: three primes, Garner
 //   t2 = (r2 - r1) p1^-1 mod p2,  t3 = (r3 - (r1 + p1 t2)) (p1 p2)^-1 mod p3,
 //   x  = r1 + p1 t2 + p1 p2 t3 < 2^89 (the exact coefficient);
@@ -886,6 +894,8 @@ __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, 
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
     constexpr int T = 1 << LT, NE = (1 << L1) * T;
     constexpr int E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
+    // packed sums need pieces of 2..8 limbs within a row of 16 lanes
+    constexpr bool PACKABLE = T >= 4 && T <= 16;
     const CrtK &K = L.crt;
     uint32_t r0[E], t2[E];
 #pragma unroll 1
@@ -933,13 +943,30 @@ __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, 
                 const uint64_t b = (s0 >> 32) + ((uint64_t)(s0 < x12) << 32) + (uint64_t)p12hi * t3;
                 const uint32_t ao = pair_swap(a);
                 const uint32_t bo_lo = pair_swap((uint32_t)b), bo_hi = pair_swap((uint32_t)(b >> 32));
-                if (even && e < NE) {
-                    const uint64_t m = b + ao;
-                    const size_t li = ((size_t)(e >> LT) << (L2 - 1)) + (C >> 1);
-                    if (li < D.sum_len) {
-                        gptr_mut(D.slo)[li] = (uint64_t)a | (m << 32);
-                        gptr_mut(D.shi)[li] = (m >> 32) + ((uint64_t)bo_hi << 32 | bo_lo);
+                const uint64_t m = b + ao;
+                const uint64_t lo = (uint64_t)a | (m << 32), hi = (m >> 32) + ((uint64_t)bo_hi << 32 | bo_lo);
+                const size_t li = ((size_t)(e >> LT) << (L2 - 1)) + (C >> 1);
+                if (PACKABLE && D.pack_log) {
+                    // limb q of a row piece (T / 2 limbs, on the even lanes
+                    // 0, 2, ..) adds the high word of limb q - 1, two lanes down
+                    const uint32_t q = (C >> 1) & (T / 2 - 1);
+                    const uint64_t hp = lane_down2(hi);
+                    const uint64_t t = q ? lo + hp : lo;
+                    // bit q: limb q's addition carried; OR-ed up the even lanes
+                    // of the piece (shifts of 2, 4, 8 lanes within a row of 16),
+                    // so the last limb's lane holds the bits of limbs 1 .. q
+                    uint32_t gb = (q && t < lo) ? 1u << q : 0u;
+                    gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x112, 0xF, 0xF, false);
+                    if constexpr (T >= 8) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x114, 0xF, 0xF, false);
+                    if constexpr (T >= 16) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x118, 0xF, 0xF, false);
+                    if (even && e < NE && li < D.sum_len) {
+                        gptr_mut(D.slo)[li] = t;
+                        if (q == T / 2 - 1)   // the piece's last limb writes its high word and carries
+                            gptr_mut(D.shi)[li >> (LT - 1)] = hi | (uint64_t)gb << 56;
                     }
+                } else if (even && e < NE && li < D.sum_len) {
+                    gptr_mut(D.slo)[li] = lo;
+                    gptr_mut(D.shi)[li] = hi;
                 }
             }
         }

@@ -502,11 +502,12 @@ struct Scratch {
 };
 
 int launch_carry(const uint64_t *slo, const uint64_t *shi, size_t m, uint64_t *out, size_t out_len,
-                 Scratch &s, int parts = 1) {
+                 Scratch &s, int parts = 1, int pack_log = 0) {
     Context &C = ctx();
     CarryLaunch cl{};
     cl.one.kind = NODE_PLAIN;
     cl.one.parts = parts;
+    cl.one.pack_log = pack_log;
     cl.one.slo = slo;
     cl.one.shi = shi;
     cl.one.m = m;
@@ -640,12 +641,17 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     if (l1 && sqr) L.one.ext[1] = L.one.spec;
     L.one.prod = s.get<uint32_t>(NP * n);
     const size_t m = n / DPL;
+    // Wide mode with column tiles of 4..16 columns: limb sums packed (pieces of
+    // T / 2 limbs, NttLeaf::pack_log; high words stay below 2^57 because every
+    // coefficient is below 2^25 (2^32 - 1)^2 < 2^89).
+    const int lt = l1 ? plan_lt(L.logn, L.variant) : 0;
+    L.one.pack_log = (mode == MODE_WIDE && lt >= 2 && lt <= 4) ? lt - 1 : 0;
     L.one.slo = s.get<uint64_t>(m);
-    L.one.shi = s.get<uint64_t>(m);
+    L.one.shi = s.get<uint64_t>(m >> L.one.pack_log);
     L.one.sum_len = m;
     if ((l1 && !L.one.spec) || !L.one.prod || !L.one.slo || !L.one.shi) return set_error(PG_ENOMEM, "NTT scratch");
     HIP_TRY(launch_ntt(L, C.stream, C.hook()));
-    return launch_carry(L.one.slo, L.one.shi, m, rp, an + bn, s);
+    return launch_carry(L.one.slo, L.one.shi, m, rp, an + bn, s, 1, L.one.pack_log);
 }
 
 }  // namespace pg

@@ -26,8 +26,9 @@ PLAIN, BLOCKED, LIN = 0, 1, 2
 
 
 class Node:
-    def __init__(self, kind, m, out_len, parts=1, B=0, blocks=(), xn=0, xsh=0, yn=0, ysh=0, flag=False):
+    def __init__(self, kind, m, out_len, parts=1, B=0, blocks=(), xn=0, xsh=0, yn=0, ysh=0, flag=False, pack_log=0):
         self.kind, self.m, self.out_len, self.parts = kind, m, out_len, parts
+        self.pack_log = pack_log
         self.B, self.blocks = B, list(blocks)
         self.xn, self.xsh, self.yn, self.ysh, self.flag = xn, xsh, yn, ysh, flag
         self.chunk0 = 0
@@ -97,7 +98,8 @@ def limb_sum_reads(N, j):
 
 
 def check_node_access(N, arr, i):
-    size = {"x": N.xn, "y": N.yn, "sums": N.m * N.parts, "out": N.out_len, "out_hi": N.out_len}.get(arr)
+    size = {"x": N.xn, "y": N.yn, "sums": N.m * N.parts, "out": N.out_len, "out_hi": N.out_len,
+            "aux": N.m >> N.pack_log}.get(arr)
     if arr.startswith("block"):
         size = N.blocks[int(arr[5:])]
     assert 0 <= i < size, (arr, i, size)
@@ -126,6 +128,9 @@ def t_reads(N, j, plain):
         if not N.m:
             return []
         jl = 0 if j < 0 else (j if j < N.m else N.m - 1)
+        if N.pack_log:   # limbs_t_packed: the limb word and its piece's (or the previous piece's) word
+            pc, q = jl >> N.pack_log, jl & ((1 << N.pack_log) - 1)
+            return [("sums", jl), ("aux", pc if (q or pc == 0) else pc - 1)]
         jh = 0 if j < 1 else (j - 1 if j - 1 < N.m else N.m - 1)
         return [("sums", jl), ("sums", jh)]
     if j < 0:
@@ -210,6 +215,16 @@ def test_single_plain_launch(m, out_len):
     # the product's carry: one plain node with whole-limb sums, no node table
     nodes = [Node(PLAIN, m, out_len)]
     check_launch(nodes, launch(nodes, chunks_batch), plain=True)
+
+
+@pytest.mark.parametrize("pack_log,m", [(1, 2), (2, 4), (2, 2048), (2, 4096), (3, 8), (3, 2048 * 3), (1, 10240)])
+def test_packed_product_launch(pack_log, m):
+    # the product's carry over packed sums (m limbs of sums, out_len <= m);
+    # the host packs only when the pieces tile the sums (m a multiple of 2^pack_log)
+    assert m % (1 << pack_log) == 0
+    for out_len in (m, m - 1):
+        nodes = [Node(PLAIN, m, out_len, pack_log=pack_log)]
+        check_launch(nodes, launch(nodes, chunks_batch), plain=True)
 
 
 def random_lin(rng, m):
