@@ -69,7 +69,10 @@ constexpr int split_lr(int l, int i, int rb = RADIX_BITS) {
 constexpr int NTT_GEOMS = 8, NTT_VARIANTS = 16, NTT_VARIANT_MIN_LOG = 12;
 constexpr int VAR_DL2[NTT_GEOMS] = {0, 0, -1, 1, -1, 1, -2, 0};
 constexpr int VAR_DLT[NTT_GEOMS] = {0, -1, 0, 0, -1, -1, 0, -2};
-constexpr int lt_for(int l1, int l2) { return cmin(cmin(6, 14 - l1), l2); }
+// Single-row plans (l1 = 0) have no column transform: their "tiles" only feed
+// the inverse column kernel's CRT, so they are as wide as its 1024 threads
+// (64-column tiles left 15 of every 16 threads idle).
+constexpr int lt_for(int l1, int l2) { return l1 == 0 ? cmin(l2, 10) : cmin(cmin(6, 14 - l1), l2); }
 constexpr bool geom_fits(int logn, int g) {
     return g > 0 && g < NTT_GEOMS && logn >= NTT_VARIANT_MIN_LOG &&
            plan_l2_default(logn) + VAR_DL2[g] >= 5 && plan_l2_default(logn) + VAR_DL2[g] <= ROW_LOG_MAX &&
