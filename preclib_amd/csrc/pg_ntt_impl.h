@@ -84,6 +84,18 @@ __device__ __forceinline__ NttLeaf leaf_of(const NttLaunch &L) {
     return u.d;
 }
 
+// A 64-bit value or pointer that is the same in every lane, in scalar
+// registers (loads through such a pointer take a scalar base and a 32-bit
+// vector offset).
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (uint64_t)hi << 32 | lo;
+}
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    return (T *)uniform_u64((uint64_t)p);
+}
+
 // 16-byte vector of the compiler's own (loads through an address-space
 // pointer need a builtin vector type; HIP's uint4 is a class)
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -180,6 +192,98 @@ __device__ __forceinline__ void pass_inv(uint32_t *lds, const uint2 *__restrict_
         dft_dit_inv<R, LZ>(x, c);
 #pragma unroll
         for (int q = 0; q < R; ++q) lds[lp(base + q * S)] = x[q];
+    }
+}
+
+// a * w mod p for a twiddle w held in Montgomery form (w 2^32 mod p), a < 2^32:
+// one Montgomery product; LZ leaves the result in [0, 2p) (then a < 2p is
+// required, so that a w < 2 p^2 < p 2^32).
+template <bool LZ>
+__device__ __forceinline__ uint32_t mul_mont_tw(uint32_t a, uint32_t wm, const PrimeK &c) {
+    const uint64_t t = (uint64_t)a * wm;
+    const uint32_t m = (uint32_t)t * c.ninv;
+    const uint32_t r = (uint32_t)((t + (uint64_t)m * c.p) >> 32);   // [0, 2p)
+    if constexpr (LZ) return r;
+    else return min(r, r - c.p);
+}
+
+// pass_inv with the pass's twiddles in LDS, in Montgomery form, laid out
+// [j2][k1] (R words per j2, k1 = 0 unused; fill_inv_tw), read 16 bytes at a
+// time.
+template <int LR, int LOGN, int LOGT, int LOGB, int NT, bool LZ>
+__device__ __forceinline__ void pass_inv_lt(uint32_t *lds, const uint32_t *tw, const PrimeK &c, int tid) {
+    constexpr int R = 1 << LR, LOGM = LOGB - LR, M = 1 << LOGM, T = 1 << LOGT;
+    constexpr int NCOLS = 1 << (LOGN - LR + LOGT);
+    constexpr int S = M * T;
+    static_assert(M == 1 || R >= 4, "pass_inv_lt: 16-byte twiddle rows");
+#pragma unroll 1
+    for (int col = tid; col < NCOLS; col += NT) {
+        const int cc = col & (T - 1), r = col >> LOGT;
+        const int blk = r >> LOGM, j2 = r & (M - 1);
+        const int base = (((blk << LOGB) + j2) << LOGT) + cc;
+        uint32_t x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = lds[lp(base + bitrev<R>(q) * S)];
+        if constexpr (M > 1) {
+            uint32_t w[R];
+#pragma unroll
+            for (int k = 0; k < R; k += 4) {
+                const u32x4 v = *(const u32x4 *)(tw + j2 * R + k);
+                w[k] = v.x; w[k + 1] = v.y; w[k + 2] = v.z; w[k + 3] = v.w;
+            }
+#pragma unroll
+            for (int q = 1; q < R; ++q) x[q] = mul_mont_tw<LZ>(x[q], w[bitrev<R>(q)], c);
+        }
+        dft_dit_inv<R, LZ>(x, c);
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[lp(base + q * S)] = x[q];
+    }
+}
+
+// LDS words of the twiddles of inverse passes I, I + 1, .. of a 2^LOGN
+// transform whose pass I has block size 2^LOGB (passes with M = 1 have none).
+constexpr int inv_tw_words(int logn, int rb, int i, int logb) {
+    return split_lr(logn, i, rb) <= 0 ? 0
+           : ((logb - split_lr(logn, i, rb)) > 0 ? 1 << logb : 0) +
+                 inv_tw_words(logn, rb, i + 1, logb - split_lr(logn, i, rb));
+}
+
+// Inverse passes I .. in LDS, their twiddles at tw + OFF, pass by pass.
+template <int LOGN, int LOGT, int NT, int LOGB, int I, int RB, bool LZ, int OFF>
+__device__ __forceinline__ void inv_from_lt(uint32_t *lds, const uint32_t *tw, const PrimeK &c, int tid) {
+    constexpr int LR = split_lr(LOGN, I, RB);
+    if constexpr (LR > 0) {
+        constexpr int M = 1 << (LOGB - LR);
+        inv_from_lt<LOGN, LOGT, NT, LOGB - LR, I + 1, RB, LZ, OFF + (M > 1 ? (1 << LOGB) : 0)>(lds, tw, c, tid);
+        pass_inv_lt<LR, LOGN, LOGT, LOGB, NT, LZ>(lds, tw + OFF, c, tid);
+        lds_sync();
+    }
+}
+
+// Copies the twiddles of inverse passes I .. of one prime from their Shoup
+// tables (itw[I] laid out [k1][j2], (w, floor(w 2^32 / p))) into LDS in
+// Montgomery form, w 2^32 mod p = -floor(w 2^32 / p) p mod 2^32 = wq (2^32 - p),
+// laid out [j2][k1] (entry k1 + R j2 from table entry k1 M + j2).
+template <int LOGN, int NT, int LOGB, int I, int RB, int OFF>
+__device__ __forceinline__ void fill_inv_tw(uint32_t *tw, const uint2 *const *itw, const PrimeK &c, int tid) {
+    constexpr int LR = split_lr(LOGN, I, RB);
+    if constexpr (LR > 0) {
+        constexpr int R = 1 << LR, M = 1 << (LOGB - LR), W = M > 1 ? R * M : 0;
+        if constexpr (W > 0) {
+            // every load of the thread issued before the first store
+            constexpr int K = (W + NT - 1) / NT;
+            const auto src = gptr(itw[I]);
+            uint32_t q[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int i = tid + k * NT;   // LDS entry k1 + R j2
+                q[k] = (W % NT == 0 || i < W) ? src[(i & (R - 1)) * M + i / R].y : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (W % NT == 0 || tid + k * NT < W) tw[OFF + tid + k * NT] = q[k] * c.np;
+        }
+        fill_inv_tw<LOGN, NT, LOGB - LR, I + 1, RB, OFF + W>(tw, itw, c, tid);
     }
 }
 
@@ -851,6 +955,68 @@ __device__ __forceinline__ void stage_prime(uint32_t *lds, const NttLaunch &L, c
     }
 }
 
+// Pipelined staging (inverse column kernel).  The tile's residues of the
+// next prime are copied by LDS DMA (global_load_lds) into a staging tile
+// B = lds + NE in plain order (element e = row * T + column at B[e]) while
+// the current prime's passes run; stage_store then applies the four-step
+// twiddles on the way from B to the transform tile.  DMA k of wave w carries
+// 64 lanes x 16 bytes, elements [256 (16 k + w), + 256): lane l holds
+// elements 4l .. 4l + 3 of that chunk, four columns of one row.
+template <int LOGN, int V>
+__device__ __forceinline__ void stage_dma(uint32_t *B, const uint32_t *prod, int pr, uint32_t c0, int tid) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int NE = (1 << L1) << LT, NW = COLINV_THREADS / 64;
+    static_assert(LT >= 2 && NE % 256 == 0, "stage_dma: rows of whole 16-byte pieces");
+    const int w = tid >> 6, l = tid & 63;
+    const uint32_t *in = prod + ((size_t)pr << LOGN);
+#pragma unroll
+    for (int k = 0; k < (NE / 256 + NW - 1) / NW; ++k) {
+        const int chunk = k * NW + w;
+        if (NE % (256 * NW) == 0 || chunk < NE / 256) {
+            const uint32_t e = (uint32_t)chunk * 256 + 4 * l;
+            const uint32_t off = ((e >> LT) << L2) + c0 + (e & ((1u << LT) - 1));
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(in + off),
+                                             (__attribute__((address_space(3))) void *)(B + chunk * 256), 16, 0, 0);
+        }
+    }
+}
+
+// Table words of the four-step twiddle and its step for prime pr (loaded
+// with the DMA, used by stage_store).
+struct StageTw {
+    uint32_t wlo, whi, slo, shi;
+};
+template <int LOGN, int V>
+__device__ __forceinline__ void stage_tw(StageTw &s, const NttLaunch &L, int pr, uint32_t c0, int tid) {
+    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
+    constexpr uint32_t NMASK = (1u << LOGN) - 1;
+    using TW = ColTw<L1, LT, RBV, COLINV_THREADS>;
+    static_assert(TW::STEP, "stage_tw: stepped twiddles only");
+    const uint32_t C = c0 + (tid & ((1 << LT) - 1));
+    const uint32_t ew = (C * freq_of_pos<L1, RBV>(tid >> LT)) & NMASK, es = (0u - (C << TW::D)) & NMASK;
+    const uint32_t lmask = (1u << L.lobits) - 1;
+    s.wlo = gptr(L.ilo[pr])[ew & lmask];
+    s.whi = gptr(L.ihi[pr])[ew >> L.lobits];
+    s.slo = gptr(L.glo[pr])[es & lmask];
+    s.shi = gptr(L.ghi[pr])[es >> L.lobits];
+}
+
+// The staged residues (B, after the DMA) times their four-step twiddles and
+// 1/n into the transform tile, as stage_prime writes them.
+template <int LOGN, int V>
+__device__ __forceinline__ void stage_store(uint32_t *lds, const uint32_t *B, const StageTw &s, const PrimeK &c,
+                                            int tid) {
+    constexpr int E = ((1 << plan_l1(LOGN, V)) << plan_lt(LOGN, V)) / COLINV_THREADS;
+    uint32_t w = mont_mul(s.wlo, s.whi, c.p, c.ninv);
+    const uint32_t st = mont_mul(s.slo, s.shi, c.p, c.ninv);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        const int e = tid + k * COLINV_THREADS;
+        lds[lp(e)] = mont_mul(B[e], w, c.p, c.ninv);
+        w = mont_mul(w, st, c.p, c.ninv);
+    }
+}
+
 // Prime pr's residues through the inverse column transform; the result is in
 // LDS (after a workgroup barrier).
 // The lazy prime's residues come out in [0, 2p).
@@ -889,14 +1055,121 @@ __device__ __forceinline__ uint64_t lane_down2(uint64_t v) {
 // (Halved tiles measured slower at 2^24 points: with 1024 threads they spill
 // at 64 VGPRs; with 512 the occupancy stays and the rows are read in 16-byte
 // pieces.)
+// Two workgroups per CU (at most 64 VGPRs) where the Garner state (2 words per
+// element) and a transform pass fit: 8 elements per thread or radix-8 passes.
 template <int LOGN, int V>
-__device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, uint32_t c0) {
+constexpr int col_inv_waves() {
+    return ((1 << (plan_l1(LOGN, V) + plan_lt(LOGN, V))) <= 8 * COLINV_THREADS || plan_rb(LOGN, V) == 3) ? 8 : 4;
+}
+
+// Whether the wide-mode inverse column kernel stages its primes by LDS DMA
+// (plans with stepped four-step twiddles whose LDS fits, in kernels that
+// hold a CU with one workgroup anyway; where two workgroups share a CU, they
+// hide each other's loads, and twice the LDS would leave room for one): its
+// LDS then holds
+// the transform tile, the staging tile and the three primes' inverse pass
+// twiddles, so its passes issue no global loads and the DMA of the next prime
+// runs under them (global loads complete in order: a twiddle load issued
+// after the DMA would wait for it).
+template <int LOGN, int V>
+constexpr int col_inv_tw_words() {
+    return inv_tw_words(plan_l1(LOGN, V), plan_rb(LOGN, V), 0, plan_l1(LOGN, V));
+}
+template <int LOGN, int V>
+constexpr size_t col_inv_pipe_lds() {
+    return (2 * ((size_t)1 << (plan_l1(LOGN, V) + plan_lt(LOGN, V))) + 3 * (size_t)col_inv_tw_words<LOGN, V>()) * 4;
+}
+template <int LOGN, int V>
+constexpr bool col_inv_pipe() {
+    return col_inv_waves<LOGN, V>() == 4 && plan_l1(LOGN, V) > 0 &&
+           ColTw<plan_l1(LOGN, V), plan_lt(LOGN, V), plan_rb(LOGN, V), COLINV_THREADS>::STEP &&
+           col_inv_pipe_lds<LOGN, V>() <= 156 * 1024;
+}
+
+// One prime's step of Garner's CRT for the tile's elements (the prime's
+// residues in LDS): prime 0 sets r0, prime 1 sets t2, prime 2 forms each
+// coefficient and writes the limb sums of the tile starting at column c0.
+template <int LOGN, int V, int E>
+__device__ __forceinline__ void crt_step(const uint32_t *lds, const NttLaunch &L, int pr, uint32_t c0, int tid,
+                                         uint32_t (&r0)[E], uint32_t (&t2)[E], uint64_t *slo, uint64_t *shi,
+                                         size_t sum_len, int pack_log) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
     constexpr int T = 1 << LT, NE = (1 << L1) * T;
-    constexpr int E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
     // packed sums need pieces of 2..8 limbs within a row of 16 lanes
     constexpr bool PACKABLE = T >= 4 && T <= 16;
     const CrtK &K = L.crt;
+    const uint32_t C = c0 + (tid & (T - 1));
+    const bool even = (tid & 1) == 0;
+    if (pr == 0) {
+        const uint32_t p1 = L.pk[0].p;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = tid + k * COLINV_THREADS;
+            const uint32_t r = e < NE ? lds[lp(e)] : 0;
+            r0[k] = min(r, r - p1);   // lazy prime: [0, 2p) -> [0, p)
+        }
+    } else if (pr == 1) {
+        const uint32_t p2 = L.pk[1].p;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = tid + k * COLINV_THREADS;
+            const uint32_t r = e < NE ? lds[lp(e)] : 0;
+            t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, L.pk[1]);
+        }
+    } else {
+        const uint32_t p3 = L.pk[2].p;
+        const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
+        const uint32_t p12lo = (uint32_t)p12, p12hi = (uint32_t)(p12 >> 32);
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = tid + k * COLINV_THREADS;
+            const uint32_t r = e < NE ? lds[lp(e)] : 0;
+            // (r1 + p1 t2) mod p3, with r1 < p1 < p3 and t2 < p2 < p3
+            const uint32_t x12m = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, L.pk[2]), p3);
+            const uint32_t t3 = mul_shoup(mod_sub(r, x12m, p3), K.inv123, K.inv123q, L.pk[2]);
+            // x = a + 2^32 b
+            const uint64_t x12 = (uint64_t)r0[k] + p1 * t2[k];
+            const uint64_t s0 = x12 + (uint64_t)p12lo * t3;
+            const uint32_t a = (uint32_t)s0;
+            const uint64_t b = (s0 >> 32) + ((uint64_t)(s0 < x12) << 32) + (uint64_t)p12hi * t3;
+            const uint32_t ao = pair_swap(a);
+            const uint32_t bo_lo = pair_swap((uint32_t)b), bo_hi = pair_swap((uint32_t)(b >> 32));
+            const uint64_t m = b + ao;
+            const uint64_t lo = (uint64_t)a | (m << 32), hi = (m >> 32) + ((uint64_t)bo_hi << 32 | bo_lo);
+            const size_t li = ((size_t)(e >> LT) << (L2 - 1)) + (C >> 1);
+            if (PACKABLE && pack_log) {
+                // limb q of a row piece (T / 2 limbs, on the even lanes
+                // 0, 2, ..) adds the high word of limb q - 1, two lanes down
+                const uint32_t q = (C >> 1) & (T / 2 - 1);
+                const uint64_t hp = lane_down2(hi);
+                const uint64_t t = q ? lo + hp : lo;
+                // bit q: limb q's addition carried; OR-ed up the even lanes
+                // of the piece (shifts of 2, 4, 8 lanes within a row of 16),
+                // so the last limb's lane holds the bits of limbs 1 .. q
+                uint32_t gb = (q && t < lo) ? 1u << q : 0u;
+                gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x112, 0xF, 0xF, false);
+                if constexpr (T >= 8) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x114, 0xF, 0xF, false);
+                if constexpr (T >= 16) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x118, 0xF, 0xF, false);
+                if (even && e < NE && li < sum_len) {
+                    gptr_mut(slo)[li] = t;
+                    if (q == T / 2 - 1)   // the piece's last limb writes its high word and carries
+                        gptr_mut(shi)[li >> (LT - 1)] = hi | (uint64_t)gb << 56;
+                }
+            } else if (even && e < NE && li < sum_len) {
+                gptr_mut(slo)[li] = lo;
+                gptr_mut(shi)[li] = hi;
+            }
+        }
+    }
+}
+
+template <int LOGN, int V>
+__device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, const NttLeaf &D, uint32_t c0) {
+    constexpr int L1 = plan_l1(LOGN, V), LT = plan_lt(LOGN, V);
+    constexpr int NE = (1 << L1) << LT, E = (NE + COLINV_THREADS - 1) / COLINV_THREADS;
+    uint64_t *const slo = uniform_ptr(D.slo), *const shi = uniform_ptr(D.shi);
+    const size_t sum_len = uniform_u64(D.sum_len);
+    const int pack_log = __builtin_amdgcn_readfirstlane(D.pack_log);
     uint32_t r0[E], t2[E];
 #pragma unroll 1
     for (int pr = 0; pr < 3; ++pr) {
@@ -905,71 +1178,62 @@ __device__ __forceinline__ void col_inv_wide(uint32_t *lds, const NttLaunch &L, 
         // the passes is recomputed instead of being held, beside the Garner
         // state, across the whole loop.
         const int tid = fresh_tid();
-        const uint32_t C = c0 + (tid & (T - 1));
-        const bool even = (tid & 1) == 0;
-        uint32_t *lds_p = lds;
-        inv_column<LOGN, V>(lds_p, L, D, pr, c0, tid);
-        if (pr == 0) {
-            const uint32_t p1 = L.pk[0].p;
+        inv_column<LOGN, V>(lds, L, D, pr, c0, tid);
+        crt_step<LOGN, V>(lds, L, pr, c0, tid, r0, t2, slo, shi, sum_len, pack_log);
+    }
+}
+
+// The pipelined form (col_inv_pipe): one workgroup per CU works through
+// tiles i = blockIdx.x, blockIdx.x + gridDim.x, .. (tile xcd_tile(i)), three
+// primes each, as one sequence of steps; the DMA of each step's residues is
+// issued while the step before it runs, the first step's alone waits.  The
+// three primes' pass twiddles are put in LDS once.
+template <int LOGN, int V>
+__device__ __forceinline__ void col_inv_wide_pipe(uint32_t *lds, const NttLaunch &L, const NttLeaf &D) {
+    constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
+    constexpr int NE = (1 << L1) << LT, E = NE / COLINV_THREADS, NTILES = (1 << L2) >> LT;
+    constexpr int TWW = col_inv_tw_words<LOGN, V>();
+    uint32_t *const B = lds + NE, *const TWL = lds + 2 * NE;
+    // descriptor fields, read once into scalar registers (read through D at
+    // each use, each is a load that later stores wait for)
+    uint64_t *const slo = uniform_ptr(D.slo), *const shi = uniform_ptr(D.shi);
+    const size_t sum_len = uniform_u64(D.sum_len);
+    const int pack_log = __builtin_amdgcn_readfirstlane(D.pack_log);
+    const uint32_t *const prod = uniform_ptr(D.prod);
+    const int nsteps = 3 * (int)((NTILES - blockIdx.x + gridDim.x - 1) / gridDim.x);
+    const auto col0 = [&](int step) {
+        return (uint32_t)xcd_tile((int)blockIdx.x + step / 3 * (int)gridDim.x, NTILES) << LT;
+    };
+    StageTw stw;
+    stage_dma<LOGN, V>(B, prod, 0, col0(0), threadIdx.x);
+    stage_tw<LOGN, V>(stw, L, 0, col0(0), threadIdx.x);
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int e = tid + k * COLINV_THREADS;
-                const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
-                r0[k] = min(r, r - p1);   // lazy prime: [0, 2p) -> [0, p)
-            }
-        } else if (pr == 1) {
-            const uint32_t p2 = L.pk[1].p;
-#pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int e = tid + k * COLINV_THREADS;
-                const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
-                t2[k] = mul_shoup(mod_sub(r, r0[k], p2), K.inv12, K.inv12q, L.pk[1]);
-            }
-        } else {
-            const uint32_t p3 = L.pk[2].p;
-            const uint64_t p1 = L.pk[0].p, p12 = p1 * L.pk[1].p;
-            const uint32_t p12lo = (uint32_t)p12, p12hi = (uint32_t)(p12 >> 32);
-#pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int e = tid + k * COLINV_THREADS;
-                const uint32_t r = e < NE ? lds_p[lp(e)] : 0;
-                // (r1 + p1 t2) mod p3, with r1 < p1 < p3 and t2 < p2 < p3
-                const uint32_t x12m = mod_add(r0[k], mul_shoup(t2[k], K.p1_mod3, K.p1_mod3q, L.pk[2]), p3);
-                const uint32_t t3 = mul_shoup(mod_sub(r, x12m, p3), K.inv123, K.inv123q, L.pk[2]);
-                // x = a + 2^32 b
-                const uint64_t x12 = (uint64_t)r0[k] + p1 * t2[k];
-                const uint64_t s0 = x12 + (uint64_t)p12lo * t3;
-                const uint32_t a = (uint32_t)s0;
-                const uint64_t b = (s0 >> 32) + ((uint64_t)(s0 < x12) << 32) + (uint64_t)p12hi * t3;
-                const uint32_t ao = pair_swap(a);
-                const uint32_t bo_lo = pair_swap((uint32_t)b), bo_hi = pair_swap((uint32_t)(b >> 32));
-                const uint64_t m = b + ao;
-                const uint64_t lo = (uint64_t)a | (m << 32), hi = (m >> 32) + ((uint64_t)bo_hi << 32 | bo_lo);
-                const size_t li = ((size_t)(e >> LT) << (L2 - 1)) + (C >> 1);
-                if (PACKABLE && D.pack_log) {
-                    // limb q of a row piece (T / 2 limbs, on the even lanes
-                    // 0, 2, ..) adds the high word of limb q - 1, two lanes down
-                    const uint32_t q = (C >> 1) & (T / 2 - 1);
-                    const uint64_t hp = lane_down2(hi);
-                    const uint64_t t = q ? lo + hp : lo;
-                    // bit q: limb q's addition carried; OR-ed up the even lanes
-                    // of the piece (shifts of 2, 4, 8 lanes within a row of 16),
-                    // so the last limb's lane holds the bits of limbs 1 .. q
-                    uint32_t gb = (q && t < lo) ? 1u << q : 0u;
-                    gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x112, 0xF, 0xF, false);
-                    if constexpr (T >= 8) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x114, 0xF, 0xF, false);
-                    if constexpr (T >= 16) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x118, 0xF, 0xF, false);
-                    if (even && e < NE && li < D.sum_len) {
-                        gptr_mut(D.slo)[li] = t;
-                        if (q == T / 2 - 1)   // the piece's last limb writes its high word and carries
-                            gptr_mut(D.shi)[li >> (LT - 1)] = hi | (uint64_t)gb << 56;
-                    }
-                } else if (even && e < NE && li < D.sum_len) {
-                    gptr_mut(D.slo)[li] = lo;
-                    gptr_mut(D.shi)[li] = hi;
-                }
-            }
+    for (int p = 0; p < 3; ++p)
+        fill_inv_tw<L1, COLINV_THREADS, L1, 0, RBV, 0>(TWL + p * TWW, L.col_itw[p], L.pk[p], threadIdx.x);
+    uint32_t r0[E], t2[E];
+#pragma unroll 1
+    for (int step = 0; step < nsteps; ++step) {
+        const int pr = step % 3;
+        // this wave's DMA of the step has landed; after the barrier every
+        // wave's has, and the previous step's CRT reads are done
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        lds_sync();
+        // (the thread index is re-read each step, so the index arithmetic
+        // is recomputed instead of being held beside the Garner state)
+        const int tid = fresh_tid();
+        stage_store<LOGN, V>(lds, B, stw, L.pk[pr], tid);
+        lds_sync();
+        // B is free: the next step's residues arrive while this one runs
+        if (step + 1 < nsteps) {
+            const int pn = (step + 1) % 3;
+            stage_dma<LOGN, V>(B, prod, pn, col0(step + 1), tid);
+            stage_tw<LOGN, V>(stw, L, pn, col0(step + 1), tid);
         }
+        if (pr == LAZY_PRIME)
+            inv_from_lt<L1, LT, COLINV_THREADS, L1, 0, RBV, true, 0>(lds, TWL + pr * TWW, L.pk[pr], tid);
+        else
+            inv_from_lt<L1, LT, COLINV_THREADS, L1, 0, RBV, false, 0>(lds, TWL + pr * TWW, L.pk[pr], tid);
+        crt_step<LOGN, V>(lds, L, pr, col0(step), tid, r0, t2, slo, shi, sum_len, pack_log);
     }
 }
 
@@ -1040,13 +1304,6 @@ __device__ __forceinline__ void col_inv_ref(uint32_t *lds, const NttLaunch &L, c
     }
 }
 
-// Two workgroups per CU (at most 64 VGPRs) where the Garner state (2 words per
-// element) and a transform pass fit: 8 elements per thread or radix-8 passes.
-template <int LOGN, int V>
-constexpr int col_inv_waves() {
-    return ((1 << (plan_l1(LOGN, V) + plan_lt(LOGN, V))) <= 8 * COLINV_THREADS || plan_rb(LOGN, V) == 3) ? 8 : 4;
-}
-
 template <int LOGN, int MODE, int V>
 __global__ void __launch_bounds__(COLINV_THREADS)
 __attribute__((amdgpu_waves_per_eu(MODE == MODE_WIDE ? col_inv_waves<LOGN, V>() : 1)))
@@ -1057,11 +1314,25 @@ k_col_inv(const NttLaunch L) {
     // a time, 208 -> 227 us at 2^24 points)
     const NttLeaf &D = L.leaves ? L.leaves[blockIdx.z] : L.one;
     const uint32_t c0 = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) << plan_lt(LOGN, V);
-    if constexpr (MODE == MODE_WIDE) col_inv_wide<LOGN, V>(lds, L, D, c0);
+    if constexpr (MODE == MODE_WIDE && col_inv_pipe<LOGN, V>()) col_inv_wide_pipe<LOGN, V>(lds, L, D);
+    else if constexpr (MODE == MODE_WIDE) col_inv_wide<LOGN, V>(lds, L, D, c0);
     else col_inv_ref<LOGN, V>(lds, L, D, c0);
 }
 
 // ---- host-side launch table ----------------------------------------------------
+
+// Compute units of the current device (looked up once per device).
+static inline int device_cus() {
+    static int cus[64] = {};
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) return 256;
+    if (!cus[d]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || n <= 0) n = 256;
+        cus[d] = n;
+    }
+    return cus[d];
+}
 
 template <int LOGN, int MODE, int V>
 static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const StageHook &h) {
@@ -1084,7 +1355,12 @@ static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const Stag
     hipLaunchKernelGGL((k_row<LOGN, MODE, V>), dim3(1 << L1, NP, nl), dim3(row_threads(LOGN, V)), row_lds, stream, L);
     h.end(h.ctx, "ntt_row", 24.0 * NP * n + (L1 ? 0.0 : io + 16.0 * NP * n));
     h.begin(h.ctx, "ntt_col_inv");
-    hipLaunchKernelGGL((k_col_inv<LOGN, MODE, V>), dim3(ntiles, 1, nl), dim3(COLINV_THREADS), col_lds, stream, L);
+    // the pipelined inverse column kernel: one workgroup per CU, each
+    // working through several tiles
+    constexpr bool PIPE = MODE == MODE_WIDE && col_inv_pipe<LOGN, V>();
+    const size_t inv_lds = PIPE ? col_inv_pipe_lds<LOGN, V>() : col_lds;
+    const int inv_grid = PIPE && device_cus() < ntiles ? device_cus() : ntiles;
+    hipLaunchKernelGGL((k_col_inv<LOGN, MODE, V>), dim3(inv_grid, 1, nl), dim3(COLINV_THREADS), inv_lds, stream, L);
     h.end(h.ctx, "ntt_col_inv", 8.0 * NP * n);
     return hipGetLastError();
 }

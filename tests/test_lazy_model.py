@@ -185,3 +185,52 @@ def test_pointwise_product_takes_lazy_operands():
     # mont_mul(a, b) needs a * b < p 2^32; lazy operands give a * b < 4 p^2
     p = 469762049
     assert (2 * p - 1) ** 2 < p << 32
+
+
+def mont_tw(x, wm, p, lazy):
+    """mul_mont_tw of pg_ntt_impl.h: one Montgomery product of x by a twiddle
+    held as wm = w 2^32 mod p, in exact 32/64-bit steps."""
+    ninv = (-pow(p, -1, 1 << 32)) % (1 << 32)
+    t = x * wm
+    assert t < 1 << 64
+    m = (t & M32) * ninv & M32
+    u = t + m * p
+    assert u < 1 << 64, "64-bit sum wraps"
+    assert u & M32 == 0
+    r = u >> 32
+    assert r < 2 * p, "Montgomery bound"
+    return r if lazy else min(r, (r - p) & M32)
+
+
+@pytest.mark.parametrize("p", sorted(PRIMES))
+def test_inverse_column_lds_twiddles(p):
+    # The inverse column kernel holds its pass twiddles in LDS as
+    # wm = wq (2^32 - p) mod 2^32, from the Shoup pair (w, wq = floor(w 2^32 / p)):
+    # that is w 2^32 mod p, and the Montgomery product by it is x w mod p,
+    # within [0, 2p) unreduced for the lazy prime's inputs x < 2p and within
+    # [0, p) after the correction for any x < 2^32.
+    rng = random.Random(p)
+    ws = [0, 1, p - 1, 2, (p - 1) // 2] + [rng.randrange(p) for _ in range(300)]
+    xs_lazy = [0, 1, p - 1, p, 2 * p - 1] + [rng.randrange(2 * p) for _ in range(20)]
+    xs_any = [0, 1, p - 1, p, M32] + [rng.randrange(1 << 32) for _ in range(20)]
+    for w in ws:
+        wq = (w << 32) // p
+        wm = wq * ((1 << 32) - p) & M32
+        assert wm == (w << 32) % p
+        assert wm < p
+        for x in xs_any:
+            assert mont_tw(x, wm, p, False) == x * w % p
+        if p < LAZY_LIMIT:
+            for x in xs_lazy:
+                r = mont_tw(x, wm, p, True)
+                assert r < 2 * p and r % p == x * w % p
+
+
+def test_inverse_column_lds_twiddles_catch_a_wrong_form():
+    # the check above rejects a plausible slip: the Shoup quotient itself
+    # (or w) taken as the Montgomery form
+    p = 1811939329
+    w = 123456789
+    wq = (w << 32) // p
+    assert mont_tw(5, wq % p, p, False) != 5 * w % p
+    assert mont_tw(5, w, p, False) != 5 * w % p
