@@ -310,20 +310,24 @@ def bench_serve(args, pgpu, workloads):
                 print("bench.py: %s: request %d differs from the same request alone" % (what, i), file=sys.stderr)
                 sys.exit(1)
 
-    # saturated throughput: every request offered at once
-    srv = pgpu.Server()
-    warm = [srv.submit(op, a, b) for op, a, b in reqs[:8]]
-    srv.drain()
-    del warm
-    t0 = time.perf_counter()
-    hs = [srv.submit(op, a, b) for op, a, b in reqs]
-    srv.drain()
-    sat_s = time.perf_counter() - t0
+    # saturated throughput: every request offered at once, three runs (one
+    # run's estimate has varied by a third between runs on one box); the
+    # median run sets the offered rates below
+    sat = []
+    for k in range(3):
+        srv = pgpu.Server()
+        warm = [srv.submit(op, a, b) for op, a, b in reqs[:8]]
+        srv.drain()
+        del warm
+        t0 = time.perf_counter()
+        hs = [srv.submit(op, a, b) for op, a, b in reqs]
+        srv.drain()
+        sat.append((time.perf_counter() - t0, srv.stats()))
+        srv.stop()
+        verify(hs, "saturated")
+        del hs
+    sat_s, st = sorted(sat, key=lambda x: x[0])[1]
     sat_rate = len(reqs) / sat_s
-    st = srv.stats()
-    srv.stop()
-    verify(hs, "saturated")
-    del hs
     levels = []
     for frac in (0.5, 0.8, 1.1):
         rate = frac * sat_rate
@@ -367,8 +371,8 @@ def bench_serve(args, pgpu, workloads):
         "config": {"workload": "serving stream, sizes log-uniform [256, 2^21], divisor >= 32768",
                    "requests": len(reqs), "divisions": sum(1 for r in reqs if r[0] == 1),
                    "host_to_host": True, "checked": "every result of every run equals the request alone"},
-        "saturated": {"req_s": sat_rate, "rounds": st["rounds"], "products": st["products"],
-                      "max_active": st["max_active"]},
+        "saturated": {"req_s": sat_rate, "runs_req_s": [len(reqs) / x[0] for x in sat], "rounds": st["rounds"],
+                      "products": st["products"], "max_active": st["max_active"]},
         "levels": levels,
     }
     print(json.dumps(line), flush=True)
