@@ -83,6 +83,7 @@ def transform_len(a, b):
 def cpu_baseline(a, b, budget_s=25.0):
     """The reference's mul_into on the same operands, one core (bounded)."""
     import ctypes
+    host = host_cpu()   # before the process pins itself to one core below
     ref = os.path.join(ROOT, "oracle", "_ref", "libpreclib_ref.so")
     P64 = ctypes.POINTER(ctypes.c_uint64)
     r = np.zeros(len(a) + len(b), np.uint64)
@@ -115,7 +116,7 @@ def cpu_baseline(a, b, budget_s=25.0):
     out = {"value": (len(a) + len(b)) / med, "unit": "product limbs/s", "cores": 1, "kind": kind,
            "sample": "%d full %dx%d-limb products (median %.2f s), pinned to one host core"
                      % (len(times), len(a), len(b), med)}
-    out.update(host_cpu())
+    out.update(host)
     return out
 
 
@@ -222,6 +223,7 @@ def bench_batch(args, pgpu, workloads):
             P64 = ctypes.POINTER(ctypes.c_uint64)
             f.restype = ctypes.c_size_t
             f.argtypes = [P64, ctypes.c_size_t, P64, ctypes.c_size_t, P64, ctypes.c_size_t, ctypes.c_int]
+            host = host_cpu()   # before pinning to one core
             try:
                 os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
             except Exception:
@@ -236,6 +238,7 @@ def bench_batch(args, pgpu, workloads):
             el = time.perf_counter() - t0
             line["cpu_baseline"] = {"value": done / el, "unit": "product limbs/s", "cores": 1, "kind": "reference",
                                     "sample": "first %d limbs of the batch in order (%.1f s), one host core" % (done, el)}
+            line["cpu_baseline"].update(host)
     print(json.dumps(line), flush=True)
     batch.free()
 
@@ -285,6 +288,7 @@ def bench_pi(args, pgpu):
             el = time.perf_counter() - t0
             line["cpu_baseline"] = {"value": sd / el, "unit": "digits/s", "cores": 1, "kind": "reference",
                                     "sample": "reference pi program, %d digits, one host core (%.1f s)" % (sd, el)}
+            line["cpu_baseline"].update(host_cpu())
     print(json.dumps(line), flush=True)
 
 
