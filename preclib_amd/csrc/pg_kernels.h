@@ -107,8 +107,13 @@ constexpr bool plan_distinct(int logn, int v) {
 // Row workgroup: 512 threads; 1024 for 2^15-point rows (32 kept values each)
 // and for radix-8 plans of 2^13 points and up (8 kept values each and every
 // thread busy in every pass, in at most 64 VGPRs: two workgroups per CU).
+// Rows shorter than 2^12 points get N2 / 8 threads (one per radix-8 column,
+// at least one wave): with 512, most threads of a short row idled in every
+// pass, and a CU held few rows (the 4096-product batch's row launches 6.96 ->
+// 3.61 ms).
 constexpr int row_threads(int logn, int v = 0) {
-    return (plan_l2(logn, v) >= 15 || (plan_rb(logn, v) == 3 && plan_l2(logn, v) >= 13)) ? 1024 : 512;
+    return (plan_l2(logn, v) >= 15 || (plan_rb(logn, v) == 3 && plan_l2(logn, v) >= 13)) ? 1024
+           : cmax(64, cmin(512, (1 << plan_l2(logn, v)) >> 3));
 }
 // Row kernel with both operands' rows in LDS (2 N2 words): for rows of up to
 // 2^13 points (64 KB, two workgroups per CU).
