@@ -230,10 +230,17 @@ double pg_server_clock(void);
  * library's table and reports ms[(logn - logn_lo) * 16 + variant] (-1 where
  * a variant equals the default) and chosen[logn - logn_lo].  The table is
  * read at pg_init from ntt_plans.txt beside libpgpu.so ("logn variant"
- * lines) or from the file $PGPU_NTT_PLANS names. */
+ * lines) or from the file $PGPU_NTT_PLANS names.
+ * A second table holds the variants of batched launches (pg_batch_plan)
+ * whose leaves hold 2^21 transform points or more in all, where throughput
+ * and not one product's latency counts ("batch logn variant" lines of the
+ * same file; -1: the single-product variant).  preclib_amd.tune --batch
+ * chooses them on the 4096-product uneven batch. */
 int pg_ntt_autotune(int logn_lo, int logn_hi, int reps, double *ms, int *chosen);
 int pg_ntt_plan_get(int *variants, int n);        /* variants[logn], logn < n */
 int pg_ntt_plan_set(const int *variants, int n);
+int pg_ntt_batch_plan_get(int *variants, int n);  /* variants[logn] or -1, logn < n */
+int pg_ntt_batch_plan_set(const int *variants, int n);
 
 /* Waits for all queued device work. */
 int pg_sync(void);

@@ -491,6 +491,26 @@ int pg_ntt_plan_set(const int *variants, int n) try {
     return pg::exception_status();
 }
 
+int pg_ntt_batch_plan_get(int *variants, int n) try {
+    LOCK;
+    NEED_INIT;
+    for (int i = 0; i < n; ++i) variants[i] = i <= NTT_MAX_LOG ? ctx().ntt_batch_tab[i] : -1;
+    return PG_OK;
+} catch (...) {
+    return pg::exception_status();
+}
+
+int pg_ntt_batch_plan_set(const int *variants, int n) try {
+    LOCK;
+    NEED_INIT;
+    for (int i = 0; i < n && i <= NTT_MAX_LOG; ++i)
+        if (variants[i] < -1 || variants[i] >= NTT_VARIANTS) return set_error(PG_EINVAL, "pg_ntt_batch_plan_set: variant");
+    for (int i = 0; i < n && i <= NTT_MAX_LOG; ++i) ctx().ntt_batch_tab[i] = i >= NTT_VARIANT_MIN_LOG ? variants[i] : -1;
+    return PG_OK;
+} catch (...) {
+    return pg::exception_status();
+}
+
 int pg_timing_enable(int on) try {
     LOCK;
     ctx().timing = on != 0;

@@ -36,6 +36,11 @@ private:
         // the same bucket; -1: transformed by this leaf
         long ext_leaf[2] = {-1, -1};
         int ext_op[2] = {0, 0};
+        // the plain carry node that alone reads this leaf's sums (a top-level
+        // product), or -1 (a block of a blocked product); pack_log: packed
+        // sums (NttLeaf::pack_log)
+        long node = -1;
+        int pack_log = 0;
     };
     struct SmallPlan {
         const uint64_t *x, *y;
@@ -51,6 +56,7 @@ private:
         size_t out_len = 0;
         SumRef out_sums;
         bool to_sums = false;
+        int pack_log = 0;
     };
     struct Bucket {
         int kind = 0, mode = 0, bits = 0, logn = 0;
@@ -75,6 +81,7 @@ private:
     char *arena = nullptr;
     size_t njobs = 0;
     std::map<std::pair<int, int>, std::vector<LeafPlan>> ntt, fft;   // (mode|bits, logn)
+    std::map<std::pair<int, int>, int> ntt_variant;                  // plan variant of each NTT bucket
     // first NTT leaf (index in its bucket, operand) transforming each
     // (mode, logn, operand pointer, length)
     std::map<std::tuple<int, int, const uint64_t *, size_t>, std::pair<long, int>> spec_owner;

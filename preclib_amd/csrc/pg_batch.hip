@@ -72,9 +72,10 @@ int Batch::plan(const uint64_t *x, size_t xn, const uint64_t *y, size_t yn, int 
                 const SumRef *target, uint64_t *final, size_t final_len) {
     const size_t lo = std::min(xn, yn), hi = std::max(xn, yn);
     // Leaves write sums; a final target needs a plain node to normalize them.
-    auto leaf_target = [&](size_t len) -> SumRef {
+    // (pad: sums allocated for len rounded up to a multiple of pad limbs)
+    auto leaf_target = [&](size_t len, size_t pad = 1) -> SumRef {
         if (target) return *target;
-        SumRef s = new_sums(len);
+        SumRef s = new_sums((len + pad - 1) / pad * pad);
         NodePlan np{};
         np.kind = NODE_PLAIN;
         np.parts = 1;
@@ -127,7 +128,9 @@ int Batch::plan(const uint64_t *x, size_t xn, const uint64_t *y, size_t yn, int 
         auto &bucket = ntt[{mode, logn}];
         LeafPlan lp{};
         lp.x = x; lp.y = y; lp.xn = xn; lp.yn = yn;
-        lp.sums = leaf_target(plen);
+        // room for whole pieces of packed sums (at most 8 limbs; build())
+        if (!target) lp.node = (long)nodes.size();
+        lp.sums = leaf_target(plen, 8);
         lp.has_spec = plan_l1(logn) > 0;
         if (lp.has_spec) {
             // An operand already transformed in this bucket (the short operand
@@ -229,6 +232,24 @@ int Batch::build(const pg_job *jobs, size_t n) {
         int st = plan(j.a, j.an, j.b, j.bn, j.path, (int)i, 0, nullptr, j.r, j.an + j.bn);
         if (st) return st;
     }
+    // The plan variant of each NTT bucket, and packed limb sums, as in the
+    // single-product path (sum_pack_log), for leaves whose sums only their own
+    // plain carry node reads.  Such a node's length is rounded up to whole
+    // pieces (at most 8 limbs, room plan() left): the sums beyond the product
+    // are exact zeros, and the node writes only out_len limbs.
+    for (auto &kv : ntt) {
+        const int mode = kv.first.first, logn = kv.first.second;
+        const int v = C.ntt_batch_variant(logn, mode, kv.second.size() << logn);
+        ntt_variant[kv.first] = v;
+        const int pl = sum_pack_log(mode, plan_lt(logn, v));
+        if (!pl) continue;
+        for (auto &lp : kv.second) {
+            if (lp.node < 0) continue;
+            NodePlan &nd = nodes[lp.node];
+            lp.pack_log = nd.pack_log = pl;
+            nd.m = (nd.m + ((size_t)1 << pl) - 1) >> pl << pl;
+        }
+    }
     // descriptor arrays live at the end of the arena
     size_t nleaf_ntt = 0, nleaf_fft = 0;
     for (auto &kv : ntt) nleaf_ntt += kv.second.size();
@@ -283,7 +304,8 @@ int Batch::build(const pg_job *jobs, size_t n) {
                     L.ext[op] = (const uint32_t *)P(kv.second[lp.ext_leaf[op]].spec) + lp.ext_op[op] * pn;
             L.prod = (uint32_t *)P(lp.prod);
             L.slo = SLO(lp.sums); L.shi = SHI(lp.sums);
-            L.sum_len = lp.xn + lp.yn;
+            L.sum_len = lp.pack_log ? nodes[lp.node].m : lp.xn + lp.yn;
+            L.pack_log = lp.pack_log;
             std::memcpy(H(d_ntt) + k * sizeof(NttLeaf), &L, sizeof L);
             b.io += (double)(lp.xn + lp.yn) * 16.0;
             ++k;
@@ -349,6 +371,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
             if (nd.kind == NODE_PLAIN) {
                 N.slo = SLO(nd.sums);
                 N.shi = SHI(nd.sums);
+                N.pack_log = nd.pack_log;
             } else {
                 N.B = nd.B;
                 N.nblocks = (int)nd.blocks.size();
@@ -395,7 +418,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
             L.io_bytes = b.io;
             L.logn = b.logn;
             L.mode = b.mode;
-            L.variant = ctx().ntt_variant(b.logn, b.mode);
+            L.variant = ntt_variant.at({b.mode, b.logn});
             L.order = ctx().ntt_order;
             int st = fill_ntt_tables(L);
             if (st) return st;

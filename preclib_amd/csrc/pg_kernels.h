@@ -19,7 +19,7 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *gptr_mut(T *p) {
     return (__attribute__((address_space(1))) T *)p;
 }
 
-constexpr int NTT_THREADS = 1024;           // workgroup size of the column kernels
+constexpr int NTT_THREADS = 1024;           // largest forward column workgroup (col_threads)
 #ifndef PG_COLINV_THREADS
 #define PG_COLINV_THREADS 1024
 #endif
@@ -38,6 +38,8 @@ constexpr size_t lds_words(size_t n) { return n; }
 constexpr int NTT_MIN_LOG = 8, NTT_MAX_LOG = 26;
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// log2 of a power of two (-1 otherwise)
+constexpr int clog2(int x) { return x <= 0 || (x & (x - 1)) ? -1 : x == 1 ? 0 : 1 + clog2(x >> 1); }
 #ifndef PG_ROW_LOG_MAX
 #define PG_ROW_LOG_MAX 15
 #endif
@@ -115,6 +117,15 @@ constexpr int row_threads(int logn, int v = 0) {
     return (plan_l2(logn, v) >= 15 || (plan_rb(logn, v) == 3 && plan_l2(logn, v) >= 13)) ? 1024
            : cmax(64, cmin(512, (1 << plan_l2(logn, v)) >> 3));
 }
+// Forward column workgroup: one thread per radix-8 column of a tile
+// (N1 T / 8), at least one wave and at least T (each thread keeps one column
+// of the tile), at most NTT_THREADS.  With 1024 threads for every tile, the
+// passes of a 2^12..2^20-point transform left half to seven eighths of the
+// threads idle (the 4096-product batch's forward column launches 4.26 ->
+// 3.16 ms); tiles of 2^13 elements and more keep 1024.
+constexpr int col_threads(int logn, int v = 0) {
+    return cmax(cmax(64, 1 << plan_lt(logn, v)), cmin(NTT_THREADS, (1 << (plan_l1(logn, v) + plan_lt(logn, v))) >> 3));
+}
 // Row kernel with both operands' rows in LDS (2 N2 words): for rows of up to
 // 2^13 points (64 KB, two workgroups per CU).
 #ifndef PG_ROW_DUAL
@@ -180,8 +191,9 @@ struct NttLeaf {
     uint32_t *prod;             // product spectra / residues [nprimes][n]
     uint64_t *slo, *shi;        // 128-bit limb sums
     size_t sum_len;             // sums written: limbs [0, sum_len)
-    // Wide mode, 2^pack_log limbs per row piece of a column tile (0: plain
-    // sums).  Packed, limb j's sum S_j = lo_j + 2^64 hi_j is written as
+    // Wide mode, 2^pack_log limbs per piece (sum_pack_log: half a row of a
+    // column tile, at most 8 limbs; 0: plain sums; sum_len then a multiple of
+    // the piece).  Packed, limb j's sum S_j = lo_j + 2^64 hi_j is written as
     // slo[j] = lo_j + hi_(j-1) mod 2^64 when j is not the first limb of its
     // piece (slo[j] = lo_j when it is), and shi[j >> pack_log] = hi of the
     // piece's last limb (< 2^57) with bit 56 + q holding the carry out of
@@ -189,6 +201,12 @@ struct NttLeaf {
     // about 10 bytes per limb instead of 16.
     int pack_log;
 };
+
+// Packed limb sums of a transform with column tiles of 2^lt columns
+// (NttLeaf::pack_log): pieces of min(2^lt / 2, 8) limbs in wide mode (the
+// carry bits of a piece's limbs 1 .. 7 sit in bits 57 .. 63 of its high
+// word), none in reference mode or for tiles of fewer than 4 columns.
+constexpr int sum_pack_log(int mode, int lt) { return (mode == MODE_WIDE && lt >= 2) ? cmin(lt, 4) - 1 : 0; }
 
 // A launch covers one leaf (`one`) or, when `leaves` is set, nleaves leaves of
 // the same transform length; blockIdx.z selects the leaf.

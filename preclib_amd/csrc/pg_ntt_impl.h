@@ -541,7 +541,7 @@ __device__ __forceinline__ uint32_t twiddle_mont(const uint32_t *__restrict__ lo
 // forming each from the tables.  Otherwise every twiddle is formed directly.
 template <int L1, int LT, int RBV, int NT>
 struct ColTw {
-    static constexpr int LNT = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : -1;
+    static constexpr int LNT = clog2(NT);
     static constexpr int E = ((1 << (L1 + LT)) + NT - 1) / NT;
     static constexpr int LR0 = split_lr(L1, 0, RBV);
     static constexpr bool STEP = LNT > 0 && E > 1 && (LNT - LT) >= (L1 - LR0);
@@ -611,15 +611,15 @@ __device__ __forceinline__ const uint32_t *op_spec(const NttLeaf &D, int op, int
 
 // Column transforms of a staged tile (half: the upper half of every column
 // is zero).
-template <int L1, int LT, int RBV, bool LZ>
+template <int L1, int LT, int RBV, int NT, bool LZ>
 __device__ __forceinline__ void col_forward(uint32_t *lds, const uint2 *const *tw, const PrimeK &c, bool half) {
     constexpr int LR0 = split_lr(L1, 0, RBV);
     if (half) {
-        pass_fwd<LR0, L1, LT, L1, NTT_THREADS, true, LZ>(lds, tw[0], c, threadIdx.x);
+        pass_fwd<LR0, L1, LT, L1, NT, true, LZ>(lds, tw[0], c, threadIdx.x);
         lds_sync();
-        fwd_from<L1, LT, NTT_THREADS, L1 - LR0, 1, RBV, MAX_PASS, LZ>(lds, tw, c, threadIdx.x);
+        fwd_from<L1, LT, NT, L1 - LR0, 1, RBV, MAX_PASS, LZ>(lds, tw, c, threadIdx.x);
     } else {
-        lds_forward<L1, LT, NTT_THREADS, RBV, MAX_PASS, LZ>(lds, tw, c);
+        lds_forward<L1, LT, NT, RBV, MAX_PASS, LZ>(lds, tw, c);
     }
 }
 
@@ -628,13 +628,15 @@ __device__ __forceinline__ void col_forward(uint32_t *lds, const uint2 *const *t
 // to the same XCD one after another (xcd_tile_prime), so the operand limbs
 // that the first reads from HBM the others find in L2.
 template <int LOGN, int MODE, int V>
-__global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(8))) k_col_fwd(const NttLaunch L) {
+__global__ void __launch_bounds__(col_threads(LOGN, V)) __attribute__((amdgpu_waves_per_eu(8))) k_col_fwd(const NttLaunch L) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V), RBV = plan_rb(LOGN, V);
     constexpr int DPL = 64 / mode_digit_bits(MODE);   // digits per limb
     constexpr int N1 = 1 << L1, T = 1 << LT, LPR = T / DPL;
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     constexpr int NP = mode_primes(MODE);
-    using TW = ColTw<L1, LT, RBV, NTT_THREADS>;
+    constexpr int NT = col_threads(LOGN, V);
+    static_assert(NT >= T, "k_col_fwd: one column per thread");
+    using TW = ColTw<L1, LT, RBV, NT>;
     extern __shared__ uint32_t lds[];
     const NttLeaf D = leaf_of(L);
     int tile, pr, op;
@@ -670,15 +672,15 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
         // grouped ahead of their LDS writes measured slower here: 32 waves per
         // CU already hide the latency, and groups of 4 spill at 64 VGPRs.)
         const int ne = (half ? N1 / 2 : N1) * LPR;
-        for (int e = threadIdx.x; e < ne; e += NTT_THREADS) {
+        for (int e = threadIdx.x; e < ne; e += NT) {
             const int j1 = e / LPR, m = e % LPR;
             const size_t li = (((size_t)j1 << L2) + c0 + DPL * m) / DPL;
             put_digits<MODE>(lds, j1 * T + DPL * m, li < nl ? src[li] : 0, L.crt.red[pr], c.p);
         }
         lds_sync();
         if constexpr (LR0 > 0) {
-            if (pr == LAZY_PRIME) col_forward<L1, LT, RBV, true>(lds, L.col_tw[pr], c, half);
-            else col_forward<L1, LT, RBV, false>(lds, L.col_tw[pr], c, half);
+            if (pr == LAZY_PRIME) col_forward<L1, LT, RBV, NT, true>(lds, L.col_tw[pr], c, half);
+            else col_forward<L1, LT, RBV, NT, false>(lds, L.col_tw[pr], c, half);
         }
         // (global stores here instead of flat ones measured slower: 282 -> 291
         // us at 2^24 points)
@@ -689,12 +691,12 @@ __global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_e
             const uint32_t st = twiddle_mont(L.glo[pr], L.ghi[pr], (C << TW::D) & NMASK, L.lobits, c);
 #pragma unroll 4
             for (int k = 0; k < TW::E; ++k) {
-                const int e = threadIdx.x + k * NTT_THREADS;
+                const int e = threadIdx.x + k * NT;
                 out[spec_off<LOGN, V>(e >> LT, C)] = mont_mul(lds[lp(e)], w, c.p, c.ninv);
                 w = mont_mul(w, st, c.p, c.ninv);
             }
         } else {
-            for (int e = threadIdx.x; e < N1 * T; e += NTT_THREADS) {
+            for (int e = threadIdx.x; e < N1 * T; e += NT) {
                 const int rho = e >> LT, cc = e & (T - 1);
                 const uint32_t ex = ((c0 + cc) * freq_of_pos<L1, RBV>(rho)) & NMASK;
                 uint32_t x = lds[lp(e)];
@@ -922,6 +924,7 @@ __device__ __forceinline__ void stage_prime(uint32_t *lds, const NttLaunch &L, c
     constexpr uint32_t NMASK = (1u << LOGN) - 1;
     using TW = ColTw<L1, LT, RBV, COLINV_THREADS>;
     const PrimeK &c = L.pk[pr];
+    static_assert(COLINV_THREADS >= T, "stage_prime: one column per thread");
     const auto in = gptr(D.prod + ((size_t)pr << LOGN));
     const uint32_t C = c0 + (tid & (T - 1));
     if constexpr (TW::STEP) {
@@ -1095,8 +1098,11 @@ __device__ __forceinline__ void crt_step(const uint32_t *lds, const NttLaunch &L
                                          size_t sum_len, int pack_log) {
     constexpr int L1 = plan_l1(LOGN, V), L2 = plan_l2(LOGN, V), LT = plan_lt(LOGN, V);
     constexpr int T = 1 << LT, NE = (1 << L1) * T;
-    // packed sums need pieces of 2..8 limbs within a row of 16 lanes
-    constexpr bool PACKABLE = T >= 4 && T <= 16;
+    // packed sums: pieces of P2 = min(T / 2, 8) limbs, within one row of 16
+    // lanes (a piece of 8 limbs is one such row: tiles start at multiples of T)
+    constexpr int PL = sum_pack_log(MODE_WIDE, LT), P2 = 1 << PL;
+    constexpr bool PACKABLE = PL > 0;
+    static_assert(COLINV_THREADS >= T, "crt_step: one column per thread");
     const CrtK &K = L.crt;
     const uint32_t C = c0 + (tid & (T - 1));
     const bool even = (tid & 1) == 0;
@@ -1138,9 +1144,9 @@ __device__ __forceinline__ void crt_step(const uint32_t *lds, const NttLaunch &L
             const uint64_t lo = (uint64_t)a | (m << 32), hi = (m >> 32) + ((uint64_t)bo_hi << 32 | bo_lo);
             const size_t li = ((size_t)(e >> LT) << (L2 - 1)) + (C >> 1);
             if (PACKABLE && pack_log) {
-                // limb q of a row piece (T / 2 limbs, on the even lanes
-                // 0, 2, ..) adds the high word of limb q - 1, two lanes down
-                const uint32_t q = (C >> 1) & (T / 2 - 1);
+                // limb q of a piece (P2 limbs, on the even lanes 0, 2, ..)
+                // adds the high word of limb q - 1, two lanes down
+                const uint32_t q = (C >> 1) & (P2 - 1);
                 const uint64_t hp = lane_down2(hi);
                 const uint64_t t = q ? lo + hp : lo;
                 // bit q: limb q's addition carried; OR-ed up the even lanes
@@ -1148,12 +1154,12 @@ __device__ __forceinline__ void crt_step(const uint32_t *lds, const NttLaunch &L
                 // so the last limb's lane holds the bits of limbs 1 .. q
                 uint32_t gb = (q && t < lo) ? 1u << q : 0u;
                 gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x112, 0xF, 0xF, false);
-                if constexpr (T >= 8) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x114, 0xF, 0xF, false);
-                if constexpr (T >= 16) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x118, 0xF, 0xF, false);
+                if constexpr (P2 >= 4) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x114, 0xF, 0xF, false);
+                if constexpr (P2 >= 8) gb |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)gb, 0x118, 0xF, 0xF, false);
                 if (even && e < NE && li < sum_len) {
                     gptr_mut(slo)[li] = t;
-                    if (q == T / 2 - 1)   // the piece's last limb writes its high word and carries
-                        gptr_mut(shi)[li >> (LT - 1)] = hi | (uint64_t)gb << 56;
+                    if (q == P2 - 1)   // the piece's last limb writes its high word and carries
+                        gptr_mut(shi)[li >> PL] = hi | (uint64_t)gb << 56;
                 }
             } else if (even && e < NE && li < sum_len) {
                 gptr_mut(slo)[li] = lo;
@@ -1348,7 +1354,7 @@ static hipError_t launch_size(const NttLaunch &L, hipStream_t stream, const Stag
         const int of = L.order & 3;
         const dim3 g = of == COLF_OP_PRIME ? dim3(ntiles, 2 * NP, nl)
                        : of == COLF_TILE_PRIME ? dim3(ntiles * NP, 2, nl) : dim3(ntiles * 2, NP, nl);
-        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE, V>), g, dim3(NTT_THREADS), col_lds, stream, L);
+        hipLaunchKernelGGL((k_col_fwd<LOGN, MODE, V>), g, dim3(col_threads(LOGN, V)), col_lds, stream, L);
         h.end(h.ctx, "ntt_col_fwd", io + 16.0 * NP * n);
     }
     h.begin(h.ctx, "ntt_row");

@@ -146,6 +146,20 @@ public:
     int ntt_variant(int logn, int mode) const {
         return (mode == MODE_WIDE && logn >= 0 && logn <= NTT_MAX_LOG) ? ntt_variant_tab[logn] : 0;
     }
+    // Plan variant of a batched launch (pg_batch) of `points` transform
+    // points in all: launches of at least BATCH_PLAN_MIN_POINTS (one 2^21-point
+    // transform's worth, about what fills the device) are throughput-bound and
+    // take the batch table ("batch logn variant" lines of the same file), tuned
+    // on the 4096-product batch; smaller ones, and sizes the batch table leaves
+    // at -1, take the single-product variant.
+    static constexpr size_t BATCH_PLAN_MIN_POINTS = (size_t)1 << 21;
+    int ntt_batch_tab[NTT_MAX_LOG + 1] = {};
+    int ntt_batch_variant(int logn, int mode, size_t points) const {
+        if (mode == MODE_WIDE && logn >= 0 && logn <= NTT_MAX_LOG && points >= BATCH_PLAN_MIN_POINTS &&
+            ntt_batch_tab[logn] >= 0)
+            return ntt_batch_tab[logn];
+        return ntt_variant(logn, mode);
+    }
     void load_plan_table();
     // workgroup order of the NTT kernels (NttLaunch::order); $PGPU_NTT_ORDER overrides
     int ntt_order = COLF_OP_PRIME | ROW_PRIME_REV;

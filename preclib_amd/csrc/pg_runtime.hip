@@ -198,9 +198,12 @@ void Context::shutdown() {
     device = -1;
 }
 
-// "logn variant" lines; unknown or invalid entries keep the default plan.
+// "logn variant" lines (single products) and "batch logn variant" lines
+// (batched launches); unknown or invalid entries keep the default plan, and
+// the single-product plan for batches.
 void Context::load_plan_table() {
     for (int &v : ntt_variant_tab) v = 0;
+    for (int &v : ntt_batch_tab) v = -1;
     std::string path;
     if (const char *e = std::getenv("PGPU_NTT_PLANS")) {
         path = e;
@@ -216,10 +219,13 @@ void Context::load_plan_table() {
     if (!f) return;
     int logn = 0, v = 0;
     char line[256];
-    while (std::fgets(line, sizeof line, f))
-        if (line[0] != '#' && std::sscanf(line, "%d %d", &logn, &v) == 2 && logn >= NTT_VARIANT_MIN_LOG &&
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        const bool batch = std::sscanf(line, "batch %d %d", &logn, &v) == 2;
+        if ((batch || std::sscanf(line, "%d %d", &logn, &v) == 2) && logn >= NTT_VARIANT_MIN_LOG &&
             logn <= NTT_MAX_LOG && v >= 0 && v < NTT_VARIANTS)
-            ntt_variant_tab[logn] = v;
+            (batch ? ntt_batch_tab : ntt_variant_tab)[logn] = v;
+    }
     std::fclose(f);
 }
 
@@ -641,11 +647,10 @@ int ntt_mul_device(uint64_t *rp, const uint64_t *ap, size_t an, uint64_t atop, c
     if (l1 && sqr) L.one.ext[1] = L.one.spec;
     L.one.prod = s.get<uint32_t>(NP * n);
     const size_t m = n / DPL;
-    // Wide mode with column tiles of 4..16 columns: limb sums packed (pieces of
-    // T / 2 limbs, NttLeaf::pack_log; high words stay below 2^57 because every
-    // coefficient is below 2^25 (2^32 - 1)^2 < 2^89).
-    const int lt = l1 ? plan_lt(L.logn, L.variant) : 0;
-    L.one.pack_log = (mode == MODE_WIDE && lt >= 2 && lt <= 4) ? lt - 1 : 0;
+    // Wide mode: limb sums packed (pieces of min(T / 2, 8) limbs,
+    // NttLeaf::pack_log; high words stay below 2^57 because every coefficient
+    // is below 2^25 (2^32 - 1)^2 < 2^89).  m is a multiple of the piece.
+    L.one.pack_log = sum_pack_log(mode, plan_lt(L.logn, mode == MODE_WIDE ? L.variant : 0));
     L.one.slo = s.get<uint64_t>(m);
     L.one.shi = s.get<uint64_t>(m >> L.one.pack_log);
     L.one.sum_len = m;

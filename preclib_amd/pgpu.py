@@ -25,7 +25,8 @@ EXPORTS = ["pg_init", "pg_shutdown", "pg_last_error", "pg_mul", "pg_buf_alloc", 
            "pg_div", "pg_div_dev", "pg_recip", "pg_recip_dev", "pg_sqrt", "pg_to_decimal",
            "pg_chud_split", "pg_pi", "pg_server_start", "pg_server_submit", "pg_server_wait",
            "pg_server_drain", "pg_server_read_stats", "pg_server_stop", "pg_server_clock",
-           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_host_staging",
+           "pg_ntt_autotune", "pg_ntt_plan_get", "pg_ntt_plan_set", "pg_ntt_batch_plan_get",
+           "pg_ntt_batch_plan_set", "pg_host_staging",
            "pg_mul_dev_static", "pg_from_decimal", "pg_debug_perturb"]
 NTT_VARIANTS = 16
 OP_MUL, OP_DIV = 0, 1
@@ -114,6 +115,8 @@ def load(path=None):
                                   ctypes.POINTER(ctypes.c_int)]
     L.pg_ntt_plan_get.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.pg_ntt_plan_set.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.pg_ntt_batch_plan_get.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.pg_ntt_batch_plan_set.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.pg_timing_enable.argtypes = [ctypes.c_int]
     L.pg_timing_read.argtypes = [ctypes.POINTER(StageTime), ctypes.c_int]
     _lib = L
@@ -315,6 +318,22 @@ def set_ntt_plans(table):
         init()
     t = (ctypes.c_int * len(table))(*table)
     _check(_lib.pg_ntt_plan_set(t, len(table)))
+
+
+def ntt_batch_plans():
+    """Plan variants of large batched launches per logn (-1: the single-product one)."""
+    if not _inited:
+        init()
+    t = (ctypes.c_int * 27)()
+    _check(_lib.pg_ntt_batch_plan_get(t, 27))
+    return list(t)
+
+
+def set_ntt_batch_plans(table):
+    if not _inited:
+        init()
+    t = (ctypes.c_int * len(table))(*table)
+    _check(_lib.pg_ntt_batch_plan_set(t, len(table)))
 
 
 def isqrt(a):

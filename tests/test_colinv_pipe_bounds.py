@@ -14,7 +14,10 @@ index arithmetic of the kernels is replayed on them:
   - every twiddle fill reads inside its R x M table and writes inside its
     pass's LDS range, each entry once; every 16-byte twiddle read of a pass
     stays inside that range;
-  - the persistent schedule visits every tile of the transform exactly once.
+  - the persistent schedule visits every tile of the transform exactly once;
+  - the forward column kernel's workgroup (col_threads) covers its tile,
+    keeps one column per thread, and its stepped twiddles follow the
+    frequencies of its rows (every plan).
 Skipped where hipcc is absent."""
 import os
 import subprocess
@@ -37,7 +40,7 @@ def plans(tmp_path_factory):
     rows = []
     for line in out[1:]:
         v = [int(x) for x in line.split()]
-        rows.append(dict(zip("logn v l1 l2 lt rb step d waves pipe lds tww".split(), v)))
+        rows.append(dict(zip("logn v l1 l2 lt rb step d waves pipe lds tww fnt fstep fd".split(), v)))
     return threads, rows
 
 
@@ -156,3 +159,48 @@ def test_persistent_schedule(plans, grid):
                 if step % 3 == 0:
                     seen.append(t)
         assert sorted(seen) == list(range(ntiles)), (r, grid)
+
+
+def freq_of_pos(rho, l1, rb):
+    """Frequency of tile row rho after the forward column passes: pass i's
+    digit (the i-th from the top of rho) becomes the i-th from the bottom."""
+    k, w, logb, i = 0, 0, l1, 0
+    while split_lr(l1, i, rb) > 0:
+        lr = split_lr(l1, i, rb)
+        logb -= lr
+        k |= ((rho >> logb) & ((1 << lr) - 1)) << w
+        w += lr
+        i += 1
+    return k
+
+
+def test_forward_column_workgroups(plans):
+    """k_col_fwd's workgroup (col_threads): whole waves, at least one column
+    per thread (a thread keeps its column C = c0 + tid mod T), the elements
+    e = tid + k NT cover the tile once; where the twiddles are stepped, the
+    frequency of a thread's k-th row is the first one's plus k 2^D, so the
+    product w = w0 * (omega^(C 2^D))^k is the element's own twiddle."""
+    _, rows = plans
+    stepped = 0
+    for r in rows:
+        l1, lt, rb, nt = r["l1"], r["lt"], r["rb"], r["fnt"]
+        if l1 == 0:
+            continue
+        ne, T = (1 << l1) << lt, 1 << lt
+        assert nt % 64 == 0 and T <= nt <= 1024 and ne % nt == 0, r
+        E = ne // nt
+        if not r["fstep"]:
+            continue
+        stepped += 1
+        assert E > 1 and r["fd"] >= 0, r
+        for tid in range(nt):
+            f0 = freq_of_pos(tid >> lt, l1, rb)
+            for k in range(E):
+                e = tid + k * nt
+                assert e & (T - 1) == tid & (T - 1)
+                assert freq_of_pos(e >> lt, l1, rb) == f0 + (k << r["fd"]), (r, tid, k)
+    assert stepped > 0
+    # short transforms run narrower workgroups; tiles of 2^13 elements and
+    # more (the 2^22-limb product's 2^24 points) keep 1024 threads
+    assert min(r["fnt"] for r in rows if r["l1"] > 0 and r["logn"] == 12) < 1024
+    assert all(r["fnt"] == 1024 for r in rows if r["l1"] + r["lt"] >= 13)

@@ -101,6 +101,48 @@ def test_device_batch_graph_replays(pg, oracle):
     batch.free()
 
 
+def test_batch_plan_table(pg, oracle):
+    """A batched launch of 2^21 transform points or more takes the batch plan
+    table (pg_ntt_batch_plan_set; -1: the single-product plan).  Every plan
+    variant of one such bucket (128 products, each a 2^14-point transform)
+    gives the oracle's products, for tiles of 16 to 64 columns (packed limb
+    sums in pieces of 8 limbs) and product lengths that are not whole pieces;
+    an out-of-range variant is refused and leaves the table as it was."""
+    single, bt = pg.ntt_plans(), pg.ntt_batch_plans()
+    rng = np.random.default_rng(77)
+    shapes = [(int(rng.integers(2100, 4097)), int(rng.integers(2100, 4097))) for _ in range(128)]
+    shapes[0] = (4096, 4096)
+    trip, want, bufs = [], [], []
+    for i, (an, bn) in enumerate(shapes):
+        a, b = W.pair(an, bn, "uniform", k=500 + i)
+        da, db, dr = pg.DevBuf.of(a), pg.DevBuf.of(b), pg.DevBuf(an + bn)
+        bufs += [da, db, dr]
+        trip.append((dr, da, an, db, bn))
+        want.append(oracle.mul_into(a, b))
+    try:
+        for v in (-1, 0, 3, 7, 13, 15):
+            t = list(bt)
+            t[14] = v
+            pg.set_ntt_batch_plans(t)
+            assert pg.ntt_batch_plans()[14] == v
+            batch = pg.DevBatch(trip)
+            for dr, _, an, _, bn in trip:
+                dr.upload(np.full(an + bn, 0xEF, np.uint64))
+            batch.run()
+            pg.sync()
+            for (dr, _, an, _, bn), w in zip(trip, want):
+                got = dr.download(an + bn)
+                assert np.array_equal(got[: len(w)], w) and not got[len(w):].any(), (v, an, bn)
+            batch.free()
+        before = pg.ntt_batch_plans()
+        with pytest.raises(pg.PgError):
+            pg.set_ntt_batch_plans([16] * 27)
+        assert pg.ntt_batch_plans() == before
+    finally:
+        pg.set_ntt_plans(single)
+        pg.set_ntt_batch_plans(bt)
+
+
 def test_shared_operand_spectra(pg, oracle):
     """Products of one device batch that share an operand (same buffer and
     length, same transform length) transform it once: a in both operand
