@@ -49,9 +49,9 @@ private:
     };
     struct NodePlan {
         int kind = NODE_PLAIN, parts = 1, depth = 0;
-        SumRef sums;                  // plain
-        std::vector<SumRef> blocks;   // blocked
-        size_t B = 0, m = 0;
+        SumRef sums, sums1;           // plain: sums; blocked: the two layers (CarryNode)
+        size_t B = 0, blen = 0, m = 0;
+        int nblocks = 0;
         uint64_t *out = nullptr;
         size_t out_len = 0;
         SumRef out_sums;

@@ -194,28 +194,37 @@ int Batch::plan(const uint64_t *x, size_t xn, const uint64_t *y, size_t yn, int 
     }
     default: break;
     }
-    // blocked (mul_unbalanced_into): blocks of 2*lo limbs of the longer operand
+    // blocked (mul_unbalanced_into): blocks of 2*lo limbs of the longer operand.
+    // Block k's product (at most 3 lo = 1.5 B limbs) is summed into layer k & 1
+    // of the node at limb k B: the blocks of one layer are 2 B apart and do not
+    // overlap, so each layer is one array of the node's length, and the carry
+    // reads limb j's sum as the two layers' words at j (where a block covers j)
+    // instead of going through per-block descriptors.
     const uint64_t *lx = xn >= yn ? x : y, *sy = xn >= yn ? y : x;
     const size_t ln = std::max(xn, yn), sn = std::min(xn, yn);
     const size_t B = 2 * sn;
     NodePlan np{};
     np.kind = NODE_BLOCKED;
     np.B = B;
+    np.blen = B + sn;
+    np.nblocks = (int)((ln + B - 1) / B);
     np.m = plen;
     np.depth = depth;
+    np.sums = new_sums(plen);
+    np.sums1 = new_sums(plen);
     if (target) { np.out_sums = *target; np.to_sums = true; }
     else { np.out = final; np.out_len = final_len; }
-    const size_t node_index = nodes.size();
     nodes.push_back(np);
-    std::vector<SumRef> blocks;
     for (size_t off = 0; off < ln; off += B) {
         const size_t pn = std::min(B, ln - off);
-        SumRef bs = new_sums(pn + sn);
-        blocks.push_back(bs);
+        const SumRef &layer = ((off / B) & 1) ? np.sums1 : np.sums;
+        SumRef bs;
+        bs.lo = layer.lo + off * 8;
+        bs.hi = layer.hi + off * 8;
+        bs.len = pn + sn;
         int st = plan(lx + off, pn, sy, sn, PG_PATH_AUTO, job, depth + 1, &bs, nullptr, 0);
         if (st) return st;
     }
-    nodes[node_index].blocks = blocks;
     return PG_OK;
 }
 
@@ -256,17 +265,13 @@ int Batch::build(const pg_job *jobs, size_t n) {
     for (auto &kv : fft) nleaf_fft += kv.second.size();
     size_t small_blocks_total = 0;
     for (auto &s : small) small_blocks_total += small_blocks(s.xn, s.yn);
-    size_t nblockrefs = 0, nchunks = 0;
+    size_t nchunks = 0;
     int maxdepth = 0;
-    for (auto &nd : nodes) {
-        nblockrefs += nd.blocks.size();
-        maxdepth = std::max(maxdepth, nd.depth);
-    }
+    for (auto &nd : nodes) maxdepth = std::max(maxdepth, nd.depth);
     const size_t d_ntt = al.take(nleaf_ntt * sizeof(NttLeaf));
     const size_t d_fft = al.take(nleaf_fft * sizeof(FftLeaf));
     const size_t d_small = al.take(small.size() * sizeof(SmallLeaf));
     const size_t d_smallmap = al.take(small_blocks_total * 4);
-    const size_t d_blockrefs = al.take(nblockrefs * sizeof(BlockRef));
     const size_t d_nodes = al.take(nodes.size() * sizeof(CarryNode));
     for (auto &nd : nodes) nchunks += (std::max(nd.m, nd.to_sums ? nd.out_sums.len : nd.out_len) + CARRY_CHUNK - 1) / CARRY_CHUNK;
     const size_t d_chunkmap = al.take(nchunks * 4);
@@ -356,7 +361,7 @@ int Batch::build(const pg_job *jobs, size_t n) {
     small_map = (uint32_t *)P(d_smallmap);
     small_nblocks = blk;
     // carry nodes, grouped by depth (deepest first)
-    size_t bref = 0, chunk = 0, nidx = 0;
+    size_t chunk = 0, nidx = 0;
     for (int d = maxdepth; d >= 0; --d) {
         Level lv;
         lv.first_chunk = chunk;
@@ -368,19 +373,16 @@ int Batch::build(const pg_job *jobs, size_t n) {
             N.kind = nd.kind;
             N.parts = nd.parts;
             N.m = nd.m;
+            N.slo = SLO(nd.sums);
+            N.shi = SHI(nd.sums);
             if (nd.kind == NODE_PLAIN) {
-                N.slo = SLO(nd.sums);
-                N.shi = SHI(nd.sums);
                 N.pack_log = nd.pack_log;
             } else {
+                N.slo1 = SLO(nd.sums1);
+                N.shi1 = SHI(nd.sums1);
                 N.B = nd.B;
-                N.nblocks = (int)nd.blocks.size();
-                N.blocks = (const BlockRef *)(P(d_blockrefs) + bref * sizeof(BlockRef));
-                for (auto &bs : nd.blocks) {
-                    BlockRef br{SLO(bs), SHI(bs), bs.len};
-                    std::memcpy(H(d_blockrefs) + bref * sizeof(BlockRef), &br, sizeof br);
-                    ++bref;
-                }
+                N.blen = nd.blen;
+                N.nblocks = nd.nblocks;
             }
             if (nd.to_sums) {
                 N.out = SLO(nd.out_sums);

@@ -53,24 +53,26 @@ __device__ __forceinline__ uint64_t shifted_limb(const uint64_t *vp, size_t n, i
 
 __device__ __forceinline__ bool lin_cond(const CarryNode &N) { return !N.cond || *gptr(N.cond) == N.cond_eq; }
 
-// Limb sum of a blocked node at block index k and offset r (limb k B + r):
-// the block products k and k - 1 that cover it.
-__device__ __forceinline__ void blocked_sum(const CarryNode &N, size_t k, size_t r, uint64_t &lo, uint64_t &hi) {
-    lo = 0;
-    hi = 0;
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-        if (k < (size_t)d) break;
-        const size_t kb = k - d;
-        if (kb >= (size_t)N.nblocks) continue;
-        const auto blk = gptr(N.blocks) + kb;
-        const size_t local = r + d * N.B;
-        if (local < blk->len) {
-            const uint64_t s = lo + gptr(blk->slo)[local];
-            hi += gptr(blk->shi)[local] + (s < lo);
-            lo = s;
-        }
-    }
+// Which layers of a blocked node hold a block product covering limb j
+// (0 <= j < m): block k = j / B (layer k & 1) whenever it exists, and block
+// k - 1 while j is below its end (k - 1 the last block: always, as j < m;
+// otherwise while j - k B < blen - B).
+__device__ __forceinline__ void blocked_cover(const CarryNode &N, size_t j, bool &c0, bool &c1) {
+    const size_t k = j / N.B, r = j - k * N.B;
+    const bool ck = k < (size_t)N.nblocks;
+    const bool cp = k >= 1 && (!ck || r < N.blen - N.B);
+    c0 = (k & 1) ? cp : ck;
+    c1 = (k & 1) ? ck : cp;
+}
+
+// Limb sum j (0 <= j < m) of a blocked node: the covering layers' words.
+__device__ __forceinline__ void blocked_sum(const CarryNode &N, size_t j, uint64_t &lo, uint64_t &hi) {
+    bool c0, c1;
+    blocked_cover(N, j, c0, c1);
+    const uint64_t l0 = c0 ? gptr(N.slo)[j] : 0, h0 = c0 ? gptr(N.shi)[j] : 0;
+    const uint64_t l1 = c1 ? gptr(N.slo1)[j] : 0, h1 = c1 ? gptr(N.shi1)[j] : 0;
+    lo = l0 + l1;
+    hi = h0 + h1 + (lo < l0);
 }
 
 // Limb sum j of a node as (lo, hi).
@@ -105,8 +107,7 @@ __device__ __forceinline__ void limb_sum(const CarryNode &N, size_t j, uint64_t 
         }
         return;
     }
-    const size_t k = j / N.B;
-    blocked_sum(N, k, j - k * N.B, lo, hi);
+    blocked_sum(N, j, lo, hi);
 }
 
 __device__ __forceinline__ uint64_t limb_t(const CarryNode &N, size_t j, uint32_t &g, uint32_t &p) {
@@ -250,24 +251,70 @@ __device__ __forceinline__ void limbs_t_plain(const CarryNode &N, long base, uin
     }
 }
 
-// Limbs t_j, j = base + 64 it (base >= 0), of a blocked node: the block
-// index j / B and offset j mod B are stepped from one limb to the next (B is
-// at least 50 limbs) instead of divided per limb.
+// 64-bit value of lane (this lane - 1); lane 0 reads 0.
+__device__ __forceinline__ uint64_t lane_up1(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+    const int src = (lane ? lane - 1 : 0) << 2;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(v >> 32));
+    return lane ? ((uint64_t)hi << 32 | lo) : 0;
+}
+
+// Lane 63's value of v (uniform).
+__device__ __forceinline__ uint64_t lane63(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return (uint64_t)hi << 32 | lo;
+}
+
+// Limbs t_j, j = base + 64 it (base >= 0, lane l of a wave at base0 + l), of
+// a blocked node.  Every lane loads both layers' words at its j (indices
+// clamped into [0, m), words no block covers discarded after), so the loads
+// of all iterations are in flight together; the high word of limb j - 1
+// comes from the lane below (lane 0: lane 63 of the previous iteration, or
+// for the first, the limb below the wave, the same for every lane).
 template <int IT>
 __device__ __forceinline__ void limbs_t_blocked(const CarryNode &N, long base, uint64_t (&t)[IT],
                                                 uint32_t (&g)[IT], uint32_t (&p)[IT]) {
-    const size_t B = N.B, m = N.m;
-    size_t k = (size_t)base / B, r = (size_t)base - k * B;
+    const long m = (long)N.m;
+    const auto a_lo = gptr(N.slo), a_hi = gptr(N.shi), b_lo = gptr(N.slo1), b_hi = gptr(N.shi1);
+    const int lane = threadIdx.x & 63;
+    uint64_t l0[IT], h0[IT], l1[IT], h1[IT];
+    if (m) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const long j = base + 64 * it;
+            const long jl = j < m ? j : m - 1;
+            l0[it] = a_lo[jl];
+            h0[it] = a_hi[jl];
+            l1[it] = b_lo[jl];
+            h1[it] = b_hi[jl];
+        }
+    }
+    uint64_t below = 0;   // high word of the limb below the wave's first
+    {
+        const long jb = base - lane - 1;
+        if (jb >= 0 && jb < m) {
+            uint64_t lo;
+            blocked_sum(N, (size_t)jb, lo, below);
+        }
+    }
+    uint64_t hprev_last = below;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const size_t j = (size_t)base + 64 * it;
-        uint64_t lo, hi, hprev = 0;
-        if (j && j <= m) {   // top word of limb j - 1's sum
-            const bool wrap = r == 0;
-            blocked_sum(N, wrap ? k - 1 : k, wrap ? B - 1 : r - 1, lo, hprev);
-        }
+        const long j = base + 64 * it;
+        uint64_t lo = 0, hi = 0;
         if (j < m) {
-            blocked_sum(N, k, r, lo, hi);
+            bool c0, c1;
+            blocked_cover(N, (size_t)j, c0, c1);
+            const uint64_t x0 = c0 ? l0[it] : 0, x1 = c1 ? l1[it] : 0;
+            lo = x0 + x1;
+            hi = (c0 ? h0[it] : 0) + (c1 ? h1[it] : 0) + (lo < x0);
+        }
+        const uint64_t up = lane_up1(hi);
+        const uint64_t hprev = lane ? up : hprev_last;   // limb j - 1 (0 beyond m)
+        hprev_last = lane63(hi);
+        if (j < m) {
             t[it] = lo + hprev;
             g[it] = t[it] < lo;
             p[it] = t[it] == ~0ull;
@@ -275,11 +322,6 @@ __device__ __forceinline__ void limbs_t_blocked(const CarryNode &N, long base, u
             t[it] = j == m ? hprev : 0;
             g[it] = 0;
             p[it] = 0;
-        }
-        r += 64;
-        while (r >= B) {
-            r -= B;
-            ++k;
         }
     }
 }
@@ -292,7 +334,7 @@ __device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint6
                                             uint32_t (&p)[IT]) {
     if (PLAIN || (N.kind == NODE_PLAIN && N.parts == 1)) {
         limbs_t_plain<IT>(N, base, t, g, p);
-    } else if (N.kind == NODE_BLOCKED && base >= 0) {
+    } else if (N.kind == NODE_BLOCKED && base - (long)(threadIdx.x & 63) >= 0) {   // (uniform per wave)
         limbs_t_blocked<IT>(N, base, t, g, p);
     } else {
 #pragma unroll

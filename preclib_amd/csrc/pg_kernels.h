@@ -310,18 +310,15 @@ constexpr int CARRY_THREADS = 256;
 constexpr int CARRY_PER_THREAD = 8;
 constexpr int CARRY_CHUNK = CARRY_THREADS * CARRY_PER_THREAD;
 
-// A block product of a blocked multiply (mul_basic.cpp:207-236): its limb
-// sums cover limbs [k*B, k*B + len) of the node's value.
-struct BlockRef {
-    const uint64_t *slo, *shi;
-    size_t len;
-};
-
 // One value to normalize.  Kinds:
 //  plain   : limb j's sum is (slo[j], shi[j]); parts == 2 when a limb is two
 //            half-limb parts (FFT tiles narrower than one limb);
 //  blocked : limb j's sum is the sum of the block products covering j (at
-//            most two: blocks are 2 lo limbs apart and at most 3 lo long).
+//            most two: blocks are B = 2 lo limbs apart and at most blen =
+//            3 lo long; mul_basic.cpp:207-236).  Block k's sums lie in layer
+//            k & 1 (slo/shi, slo1/shi1: arrays of m limbs) at limb k B, so
+//            limb j's sum is the layers' words at j where a block covers j;
+//            elsewhere the layers are not written.
 // The result goes to out[0, out_len) as limbs; with out_hi set it is written
 // as sums (out_hi zeroed) for a parent blocked node.
 //  linear  : value = xmul * X' + 2^pow2 + (yneg ? ~Y' : Y') + cin over m
@@ -336,9 +333,9 @@ enum { NODE_PLAIN = 0, NODE_BLOCKED = 1, NODE_LIN = 2 };
 struct CarryNode {
     int kind, parts;
     const uint64_t *slo, *shi;
-    const BlockRef *blocks;
+    const uint64_t *slo1, *shi1;   // blocked: the odd blocks' layer
     int nblocks;
-    size_t B;
+    size_t B, blen;
     size_t m;                   // limbs covered by the sums
     uint64_t *out, *out_hi;
     size_t out_len;

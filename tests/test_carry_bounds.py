@@ -7,14 +7,22 @@ start at chunk0 = the sum of the chunks before it, and a node covers
 ceil(max(m (+1 with a flag word), out_len) / CHUNK) chunks): every chunk,
 wave, lane and iteration, with the kernel constants (CHUNK = 2048 limbs,
 8 groups of 64 per wave, 4 waves).  Each read and write must fall inside its
-array: the limb sums (m entries, 2 m for half-limb parts), the blocks of a
-blocked node (len entries each), the linear operands (xn / yn), the output
+array: the limb sums (m entries, 2 m for half-limb parts), the two layers of
+a blocked node (m entries each), the linear operands (xn / yn), the output
 (out_len), the chunk map and aggregates (nchunks); the fix-up's look-back
 reads only aggregates below its chunk and ends (at the latest at its node's
 first chunk, which the first pass always resolves).  Node-table launches
 read plain whole-limb nodes with the product form's clamped loads, and
-blocked nodes with a block index and offset stepped from limb to limb; the
-stepping is checked against division at every limb it visits.
+blocked nodes with both layers' clamped loads at every lane's limb plus the
+limb below the wave.
+
+The blocked layout itself is checked on values: block k of a blocked
+multiply (Batch::plan) is summed into layer k & 1 at limb k B, the blocks of
+a layer never overlap, and limb j's sum read through blocked_cover equals the
+sum of the block products covering j, whatever the unwritten words hold; the
+loader's lanes (the high word of limb j - 1 from the lane below, lane 63 of
+the previous iteration, or the limb below the wave) give the same t, g, p as
+limb-by-limb arithmetic.
 """
 import random
 
@@ -26,10 +34,11 @@ PLAIN, BLOCKED, LIN = 0, 1, 2
 
 
 class Node:
-    def __init__(self, kind, m, out_len, parts=1, B=0, blocks=(), xn=0, xsh=0, yn=0, ysh=0, flag=False, pack_log=0):
+    def __init__(self, kind, m, out_len, parts=1, B=0, blen=0, nblocks=0, xn=0, xsh=0, yn=0, ysh=0, flag=False,
+                 pack_log=0):
         self.kind, self.m, self.out_len, self.parts = kind, m, out_len, parts
         self.pack_log = pack_log
-        self.B, self.blocks = B, list(blocks)
+        self.B, self.blen, self.nblocks = B, blen, nblocks
         self.xn, self.xsh, self.yn, self.ysh, self.flag = xn, xsh, yn, ysh, flag
         self.chunk0 = 0
 
@@ -83,40 +92,37 @@ def limb_sum_reads(N, j):
         if N.parts == 1:
             return [("sums", j)]
         return [("sums", 2 * j), ("sums", 2 * j + 1)]
-    out = []
-    k = j // N.B
-    for d in (0, 1):
-        if k < d:
-            break
-        kb = k - d
-        if kb >= len(N.blocks):
-            continue
-        local = j - kb * N.B
-        if local < N.blocks[kb]:
-            out.append(("block%d" % kb, local))
-    return out
+    c0, c1 = blocked_cover(N.B, N.blen, N.nblocks, j)
+    return ([("l0", j)] if c0 else []) + ([("l1", j)] if c1 else [])
+
+
+def blocked_cover(B, blen, nblocks, j):
+    """blocked_cover (pg_carry.hip): the layers whose block covers limb j."""
+    k, r = divmod(j, B)
+    ck = k < nblocks
+    cp = k >= 1 and (not ck or r < blen - B)
+    return (cp, ck) if k & 1 else (ck, cp)
 
 
 def check_node_access(N, arr, i):
     size = {"x": N.xn, "y": N.yn, "sums": N.m * N.parts, "out": N.out_len, "out_hi": N.out_len,
-            "aux": N.m >> N.pack_log}.get(arr)
-    if arr.startswith("block"):
-        size = N.blocks[int(arr[5:])]
+            "aux": N.m >> N.pack_log, "l0": N.m, "l1": N.m}.get(arr)
     assert 0 <= i < size, (arr, i, size)
 
 
-def blocked_steps(N, base, iters):
-    """(k, r) of limbs base + 64 it as limbs_t_blocked steps them (one
-    division, then r += 64 with carries into k); each must equal divmod."""
-    k, r = divmod(base, N.B)
+def blocked_wave_reads(N, base0, iters):
+    """limbs_t_blocked for the wave whose lane l starts at limb base0 + l:
+    both layers at min(j, m - 1) for every lane and iteration, and the limb
+    below the wave (blocked_sum) when it is in [0, m)."""
     out = []
-    for it in range(iters):
-        assert (k, r) == divmod(base + 64 * it, N.B)
-        out.append((k, r))
-        r += 64
-        while r >= N.B:
-            r -= N.B
-            k += 1
+    if N.m:
+        for it in range(iters):
+            for lane in range(64):
+                jl = min(base0 + lane + 64 * it, N.m - 1)
+                out += [("l0", jl), ("l1", jl)]
+    jb = base0 - 1
+    if 0 <= jb < N.m:
+        out += limb_sum_reads(N, jb)
     return out
 
 
@@ -158,13 +164,14 @@ def spec_accesses(nodes, cmap, plain, reads=t_reads):
             base = cbase + wave * 64 * ITERS + np.arange(64)
             window = wave == 0 and not start
             b0 = base - 64 if window else base
-            if N.kind == BLOCKED and not plain:
-                for lane_base in b0.tolist():
-                    blocked_steps(N, lane_base, ITERS + (1 if window else 0))
-            for it in range(ITERS + (1 if window else 0)):
-                for j in (b0 + 64 * it).tolist():
-                    for a, i in reads(N, j, plain):
-                        check_node_access(N, a, i)
+            if N.kind == BLOCKED and not plain and reads is t_reads:
+                for a, i in blocked_wave_reads(N, int(b0[0]), ITERS + (1 if window else 0)):
+                    check_node_access(N, a, i)
+            else:
+                for it in range(ITERS + (1 if window else 0)):
+                    for j in (b0 + 64 * it).tolist():
+                        for a, i in reads(N, j, plain):
+                            check_node_access(N, a, i)
             for it in range(ITERS):
                 for j in (base + 64 * it).tolist():
                     for a, i in writes(N, j):
@@ -194,8 +201,13 @@ def fix_accesses(nodes, cmap, agg_resolved, plain):
                     break
                 top -= 64
             cbase = (chunk - N.chunk0) * CHUNK
+            if N.kind == BLOCKED and not plain:
+                for grp in range(CHUNK // 64):
+                    for a, i in blocked_wave_reads(N, cbase + 64 * grp, 1):
+                        check_node_access(N, a, i)
             for j in range(cbase, cbase + CHUNK):
-                for a, i in t_reads(N, j, plain) + writes(N, j):
+                reads = [] if (N.kind == BLOCKED and not plain) else t_reads(N, j, plain)
+                for a, i in reads + writes(N, j):
                     check_node_access(N, a, i)
 
 
@@ -241,9 +253,8 @@ def random_batch_node(rng, m):
     sn = rng.randrange(1, 800)
     ln = rng.randrange(sn, 6 * sn + 1)
     B = 2 * sn
-    blocks = [min(B, ln - off) + sn for off in range(0, ln, B)]
     plen = ln + sn
-    return Node(BLOCKED, plen, plen + rng.choice([0, 0, 1]), B=B, blocks=blocks)
+    return Node(BLOCKED, plen, plen + rng.choice([0, 0, 1]), B=B, blen=B + sn, nblocks=(ln + B - 1) // B)
 
 
 SIZES = [1, 2, 63, 64, 65, 2047, 2048, 2049, 4095]
@@ -273,3 +284,120 @@ def test_model_catches_an_unclamped_window():
     nodes = [Node(PLAIN, 4096, 4097)]
     with pytest.raises(AssertionError):
         spec_accesses(nodes, launch(nodes, chunks_batch), True, reads=unclamped)
+
+
+M64 = (1 << 64) - 1
+
+
+def blocked_layers(rng, ln, sn):
+    """Batch::plan's blocked layout with random block products: block k
+    (pn + sn limbs of 128-bit sums below 2^121) summed into layer k & 1 at
+    limb k B; unwritten words hold garbage.  Returns the layers, the node and
+    the exact limb sums (the blocks covering each limb added)."""
+    B = 2 * sn
+    m = ln + sn
+    nblocks = (ln + B - 1) // B
+    layers = [[rng.getrandbits(128) | (1 << 127) for _ in range(m)] for _ in range(2)]
+    written = [set(), set()]
+    exact = [0] * m
+    for off in range(0, ln, B):
+        k = off // B
+        pn = min(B, ln - off)
+        for i in range(pn + sn):
+            v = rng.getrandbits(121)
+            assert off + i not in written[k & 1], "blocks of one layer overlap"
+            written[k & 1].add(off + i)
+            layers[k & 1][off + i] = v
+            exact[off + i] += v
+    return layers, Node(BLOCKED, m, m, B=B, blen=B + sn, nblocks=nblocks), exact
+
+
+def layer_sum(layers, N, j, cover=blocked_cover):
+    c0, c1 = cover(N.B, N.blen, N.nblocks, j)
+    return (layers[0][j] if c0 else 0) + (layers[1][j] if c1 else 0)
+
+
+@pytest.mark.parametrize("ln,sn", [(3, 1), (4, 2), (5, 2), (100, 50), (101, 50), (149, 50), (150, 50),
+                                   (199, 50), (200, 50), (201, 50), (4000, 1000), (3999, 1000), (4001, 999)])
+def test_blocked_layers_sum(ln, sn):
+    """Every limb's sum read through blocked_cover equals the sum of the block
+    products that cover it (last block full or short, one block or many)."""
+    rng = random.Random(ln * 7 + sn)
+    layers, N, exact = blocked_layers(rng, ln, sn)
+    for j in range(N.m):
+        assert layer_sum(layers, N, j) == exact[j], j
+
+
+def test_blocked_layers_catch_an_off_by_one():
+    """Covering block k - 1 while r <= blen - B (one limb too far) reads a
+    word of the next block of that layer's gap: the sums then differ."""
+    def cover_slip(B, blen, nblocks, j):
+        k, r = divmod(j, B)
+        ck = k < nblocks
+        cp = k >= 1 and (not ck or r <= blen - B)
+        return (cp, ck) if k & 1 else (ck, cp)
+    rng = random.Random(3)
+    layers, N, exact = blocked_layers(rng, 400, 50)
+    assert any(layer_sum(layers, N, j, cover_slip) != exact[j] for j in range(N.m))
+
+
+def t_g_p(S, j):
+    """Limb-by-limb t_j = lo_j + hi_(j-1) (mod 2^64), generate and propagate,
+    of limb sums S (the top word t_m = hi_(m-1))."""
+    m = len(S)
+    hprev = (S[j - 1] >> 64) if 1 <= j <= m else 0
+    if 0 <= j < m:
+        s = (S[j] & M64) + hprev
+        t = s & M64
+        return t, s >> 64, int(t == M64)
+    return (hprev if j == m else 0), 0, 0
+
+
+def blocked_wave(layers, N, base0, iters):
+    """limbs_t_blocked's lanes for the wave at limb base0 + lane: each lane's
+    (lo, hi) from the covered layer words; limb j - 1's high word from the
+    lane below, lane 0 from lane 63 of the previous iteration or, first, the
+    limb below the wave."""
+    m = N.m
+    jb = base0 - 1
+    below = (layer_sum(layers, N, jb) >> 64) if 0 <= jb < m else 0
+    last = below
+    out = []
+    for it in range(iters):
+        los, his = [], []
+        for lane in range(64):
+            j = base0 + lane + 64 * it
+            if j < m:
+                c0, c1 = blocked_cover(N.B, N.blen, N.nblocks, j)
+                w0 = layers[0][min(j, m - 1)] if c0 else 0
+                w1 = layers[1][min(j, m - 1)] if c1 else 0
+                x0, x1 = w0 & M64, w1 & M64
+                lo = (x0 + x1) & M64
+                hi = ((w0 >> 64) + (w1 >> 64) + ((x0 + x1) >> 64)) & M64
+            else:
+                lo, hi = 0, 0
+            los.append(lo)
+            his.append(hi)
+        for lane in range(64):
+            j = base0 + lane + 64 * it
+            hprev = his[lane - 1] if lane else last
+            if j < m:
+                s = los[lane] + hprev
+                out.append((j, (s & M64, s >> 64, int((s & M64) == M64))))
+            else:
+                out.append((j, (hprev if j == m else 0, 0, 0)))
+        last = his[63]
+    return out
+
+
+@pytest.mark.parametrize("ln,sn", [(150, 50), (4001, 999), (5000, 1200), (300, 2)])
+def test_blocked_loader_lanes(ln, sn):
+    """The loader's cross-lane high words give limb-by-limb t, g, p for every
+    wave of every chunk (windows included), up to limb m + 64."""
+    rng = random.Random(ln + sn)
+    layers, N, exact = blocked_layers(rng, ln, sn)
+    for base0 in list(range(0, N.m + 64, 512)) + [CHUNK - 64, N.m - 3]:
+        if base0 < 0:
+            continue
+        for j, tgp in blocked_wave(layers, N, base0, ITERS + 1):
+            assert tgp == t_g_p(exact, j), (base0, j)
