@@ -16,7 +16,10 @@ int chud_split(size_t nterms, DBuf &P, size_t &lp, DBuf &Q, size_t &lq, DBuf &T,
 int isqrt(DBuf &x, size_t &xl, const uint64_t *a, size_t an);
 // Decimal digits of x, most significant first, no leading zeros; empty for
 // zero (precn_base_convert with base 10, src/base_convert.cpp:107-172).
-int to_decimal(const uint64_t *x, size_t xn, std::string &out);
+// point > 0: formatted as the pi program prints x / 10^point instead
+// (src/pi_chudnovsky.cpp:186-192): at least one integer digit, '.', then
+// `point` digits ("0" for zero before the padding).
+int to_decimal(const uint64_t *x, size_t xn, std::string &out, size_t point = 0);
 // The value of a string of n decimal digits ('0'..'9' only, most significant
 // first), trimmed limbs (empty for zero): precn_t(std::string),
 // src/init_basic.cpp:66-92.

@@ -28,7 +28,8 @@ namespace pg {
 
 constexpr uint64_t CHUD_C3_OVER_24 = 10939058860032000ull;   // pi_chudnovsky.cpp:11
 constexpr uint64_t CHUD_A = 13591409u, CHUD_B = 545140134u;   // 12-13
-constexpr int CHUD_THREAD_LEVELS = 5;                         // per-thread levels 1..5
+constexpr int CHUD_THREAD_LEVELS = 4;                         // per-thread levels 1..4
+constexpr int CHUD_DEVICE_LEVELS = 8;                         // one wave per merge: levels 5..8
 constexpr uint64_t TEN19 = 10000000000000000000ull;
 constexpr int DEC_LEAF = 32;                                  // chunks per conversion thread
 
@@ -168,6 +169,175 @@ __global__ void __launch_bounds__(256) k_chud_merge(ChudLevel in, ChudLevel out)
     out.lp[i] = t_len(P, llp + rlp); out.lq[i] = t_len(Q, llq + rlq); out.lt[i] = t_len(T, tn);
 }
 
+// ---- one wave per merge (levels CHUD_THREAD_LEVELS + 1 .. CHUD_DEVICE_LEVELS) ----------
+//
+// A level-j node has at most 3 * 2^j limbs (slot): one thread per merge left
+// most of the device idle from level 5 on (88 workgroups), and host-planned
+// batches of tens of thousands of small products cost more on the host than
+// on the GPU.  Here a wave forms each product as column sums: lane l owns
+// column c = 64 k + l of chunk k and sums the 128-bit products a_i b_(c-i)
+// in three words; one lane then turns the columns into limbs.
+
+// 192-bit s += (x0, x1, x2)
+__device__ __forceinline__ void add192(uint64_t *s, uint64_t x0, uint64_t x1, uint64_t x2) {
+    const uint64_t y0 = s[0] + x0;
+    const uint64_t c0 = y0 < x0;
+    uint64_t y1 = s[1] + x1;
+    uint64_t c1 = y1 < x1;
+    y1 += c0;
+    c1 += y1 < c0;
+    s[0] = y0;
+    s[1] = y1;
+    s[2] += x2 + c1;
+}
+
+// Column sums of a * b (a, b in LDS, na, nb >= 1) added into sums[3 c ..
+// 3 c + 2], c < na + nb - 1.
+__device__ void wave_cols(uint64_t *sums, const uint64_t *a, uint32_t na, const uint64_t *b, uint32_t nb,
+                          uint32_t lane) {
+    const uint32_t ncols = na + nb - 1;
+    for (uint32_t c0 = 0; c0 < ncols; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        uint64_t s0 = 0, s1 = 0, s2 = 0;
+        // the i some lane of the chunk uses: c - (nb - 1) <= i <= c
+        const uint32_t ilo = c0 + 1 > nb ? c0 + 1 - nb : 0;
+        const uint32_t ihi = min(na - 1, c0 + 63);
+        for (uint32_t i = ilo; i <= ihi; ++i) {
+            const uint32_t j = c - i;   // (wraps when i > c: then j >= nb)
+            if (i <= c && j < nb) {
+                const uint64_t x = a[i], y = b[j];
+                const uint64_t lo = x * y, hi = __umul64hi(x, y);
+                s0 += lo;
+                const uint64_t k0 = s0 < lo;
+                s1 += hi;
+                const uint64_t k1 = s1 < hi;
+                s1 += k0;
+                s2 += k1 + (s1 < k0);
+            }
+        }
+        if (c < ncols) add192(sums + 3 * (size_t)c, s0, s1, s2);
+    }
+}
+
+// 64-bit value of lane (this lane - 1); lane 0 reads 0.
+__device__ __forceinline__ uint64_t lane_below(uint64_t v, uint32_t lane) {
+    const int src = (int)(lane ? lane - 1 : 0) << 2;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(v >> 32));
+    return lane ? ((uint64_t)hi << 32 | lo) : 0;
+}
+
+// dst[0, n) = sum over c < ncols of sums[c] 2^(64 c) (the value fits n
+// limbs, n >= ncols); returns the trimmed length.  64 limbs at a time: limb c
+// gathers w0 of column c, w1 of column c - 1 and w2 of column c - 2 as
+// lo + 2^64 e (e <= 2), takes e of the limb below, x = lo + e_below =
+// t + 2^64 g, and the carries into the 64 limbs follow from the generate (g)
+// and propagate (t = 2^64 - 1) masks by one integer addition: with
+// A = G | P and B = G, the carry into bit i of A + B + cin is bit i of
+// (A + B + cin) ^ P.
+__device__ uint32_t wave_norm(uint64_t *dst, uint32_t n, const uint64_t *sums, uint32_t ncols, uint32_t lane) {
+    uint64_t e_in = 0;   // e of the limb below the chunk
+    uint64_t cin = 0;    // carry into the chunk's first limb
+    uint32_t top = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t c = c0 + lane;
+        uint64_t lo = 0, e = 0;
+        if (c < n) {
+            const uint64_t a = c < ncols ? sums[3 * (size_t)c] : 0;
+            const uint64_t b = (c >= 1 && c - 1 < ncols) ? sums[3 * (size_t)(c - 1) + 1] : 0;
+            const uint64_t d = (c >= 2 && c - 2 < ncols) ? sums[3 * (size_t)(c - 2) + 2] : 0;
+            lo = a + b;
+            e = lo < b;
+            lo += d;
+            e += lo < d;
+        }
+        const uint64_t below = lane_below(e, lane);
+        const uint64_t eb = lane ? below : e_in;
+        const uint64_t t = lo + eb;
+        const uint64_t G = __ballot(t < eb), Pm = __ballot(t == ~0ull);
+        const uint64_t Cv = (((G | Pm) + G) + cin) ^ Pm;
+        const uint64_t r = t + ((Cv >> lane) & 1);
+        if (c < n) dst[c] = r;
+        const uint64_t nz = __ballot(c < n && r != 0);
+        if (nz) top = c0 + 64 - (uint32_t)__clzll((long long)nz);
+        cin = ((G >> 63) & 1) | ((Pm >> 63) & (Cv >> 63) & 1);
+        e_in = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e, 63);
+    }
+    return top;
+}
+
+// Copies n words global -> LDS by the wave.
+__device__ __forceinline__ void wave_stage(uint64_t *dst, const uint64_t *src, uint32_t n, uint32_t lane) {
+    for (uint32_t k = lane; k < n; k += 64) dst[k] = src[k];
+}
+
+// Zeroes n words of LDS.
+__device__ __forceinline__ void wave_zero(uint64_t *dst, uint32_t n, uint32_t lane) {
+    for (uint32_t k = lane; k < n; k += 64) dst[k] = 0;
+}
+
+// out = x * y (+ u * v when u is set) into n limbs; returns the trimmed length.
+// LDS: A, B (slot words each), S (3 (2 slot + 1) words).
+__device__ uint32_t wave_product(uint64_t *out, uint32_t n, const uint64_t *x, uint32_t xn, const uint64_t *y,
+                                 uint32_t yn, const uint64_t *u, uint32_t un, const uint64_t *v, uint32_t vn,
+                                 uint64_t *A, uint64_t *B, uint64_t *S, uint32_t lane) {
+    wave_zero(S, 3 * n, lane);
+    if (xn && yn) {
+        wave_stage(A, x, xn, lane);
+        wave_stage(B, y, yn, lane);
+        __syncthreads();
+        wave_cols(S, A, xn, B, yn, lane);
+        __syncthreads();
+    }
+    if (u && un && vn) {
+        wave_stage(A, u, un, lane);
+        wave_stage(B, v, vn, lane);
+        __syncthreads();
+        wave_cols(S, A, un, B, vn, lane);
+    }
+    __syncthreads();
+    const uint32_t len = wave_norm(out, n, S, n, lane);
+    __syncthreads();   // S and A, B are reused by the next product
+    return len;
+}
+
+// Level j -> j + 1 with one 64-thread workgroup per merge (the arithmetic of
+// k_chud_merge): Q = lq rq, P = lp rp, T = lt rq + rt lp; an unpaired last
+// node moves up unchanged.  Dynamic LDS: (2 slot + 3 (2 slot + 1)) words of
+// the input level.
+__global__ void __launch_bounds__(64) k_chud_merge_wave(ChudLevel in, ChudLevel out) {
+    extern __shared__ uint64_t sh[];
+    const uint32_t i = blockIdx.x, lane = threadIdx.x;
+    if (i >= out.count) return;
+    const uint32_t l = 2 * i, r = l + 1;
+    uint64_t *P = out.P + (size_t)i * out.slot, *Q = out.Q + (size_t)i * out.slot, *T = out.T + (size_t)i * out.slot;
+    const uint64_t *lP = in.P + (size_t)l * in.slot, *lQ = in.Q + (size_t)l * in.slot, *lT = in.T + (size_t)l * in.slot;
+    const uint32_t llp = in.lp[l], llq = in.lq[l], llt = in.lt[l];
+    if (r >= in.count) {
+        for (uint32_t k = lane; k < llp; k += 64) P[k] = lP[k];
+        for (uint32_t k = lane; k < llq; k += 64) Q[k] = lQ[k];
+        for (uint32_t k = lane; k < llt; k += 64) T[k] = lT[k];
+        if (lane == 0) {
+            out.lp[i] = llp;
+            out.lq[i] = llq;
+            out.lt[i] = llt;
+        }
+        return;
+    }
+    const uint64_t *rP = in.P + (size_t)r * in.slot, *rQ = in.Q + (size_t)r * in.slot, *rT = in.T + (size_t)r * in.slot;
+    const uint32_t rlp = in.lp[r], rlq = in.lq[r], rlt = in.lt[r];
+    uint64_t *A = sh, *B = sh + in.slot, *S = sh + 2 * (size_t)in.slot;
+    const uint32_t q = wave_product(Q, llq + rlq, lQ, llq, rQ, rlq, nullptr, 0, nullptr, 0, A, B, S, lane);
+    const uint32_t p = wave_product(P, llp + rlp, lP, llp, rP, rlp, nullptr, 0, nullptr, 0, A, B, S, lane);
+    const uint32_t tn = (llt + rlq > rlt + llp ? llt + rlq : rlt + llp) + 1;
+    const uint32_t t = wave_product(T, tn, lT, llt, rQ, rlq, rT, rlt, lP, llp, A, B, S, lane);
+    if (lane == 0) {
+        out.lp[i] = p;
+        out.lq[i] = q;
+        out.lt[i] = t;
+    }
+}
+
 struct LenReq {
     const uint64_t *p;
     uint32_t n;
@@ -228,11 +398,11 @@ int chud_split(size_t nterms, DBuf &Pout, size_t &lp, DBuf &Qout, size_t &lq, DB
     lp = lq = lt = 0;
     if (!nterms) return set_error(PG_EINVAL, "binary splitting needs at least one term");
     if (nterms > ((size_t)1 << 20)) return set_error(PG_ERANGE, "binary splitting limited to 2^20 terms");
-    // per-thread levels
-    ChudLevel lv[CHUD_THREAD_LEVELS + 1]{};
+    // per-thread levels, then one wave per merge
+    ChudLevel lv[CHUD_DEVICE_LEVELS + 1]{};
     size_t count = (nterms + 1) / 2;
     int st;
-    for (int j = 1; j <= CHUD_THREAD_LEVELS; ++j) {
+    for (int j = 1; j <= CHUD_DEVICE_LEVELS; ++j) {
         ChudLevel &L = lv[j];
         L.count = (uint32_t)count;
         L.slot = 3u << j;
@@ -247,18 +417,22 @@ int chud_split(size_t nterms, DBuf &Pout, size_t &lp, DBuf &Qout, size_t &lq, DB
         C.stage_begin(e0);
         if (j == 1)
             hipLaunchKernelGGL(k_chud_base, dim3(blocks_for(count)), dim3(256), 0, C.stream, (uint64_t)nterms, L);
-        else
+        else if (j <= CHUD_THREAD_LEVELS)
             hipLaunchKernelGGL(k_chud_merge, dim3(blocks_for(count)), dim3(256), 0, C.stream, lv[j - 1], L);
-        C.stage_end("chud_thread", e0, 0.0);
+        else
+            hipLaunchKernelGGL(k_chud_merge_wave, dim3((unsigned)count), dim3(64),
+                               (2 * (size_t)lv[j - 1].slot + 3 * (2 * (size_t)lv[j - 1].slot + 1)) * 8, C.stream,
+                               lv[j - 1], L);
+        C.stage_end(j <= CHUD_THREAD_LEVELS ? "chud_thread" : "chud_wave", e0, 0.0);
         HIP_TRY(hipGetLastError());
-        if (count == 1 && j < CHUD_THREAD_LEVELS) {   // done early: carry the root up
-            lv[CHUD_THREAD_LEVELS] = L;
+        if (count == 1 && j < CHUD_DEVICE_LEVELS) {   // done early: carry the root up
+            lv[CHUD_DEVICE_LEVELS] = L;
             break;
         }
         count = (count + 1) / 2;
     }
-    // nodes of the last per-thread level
-    const ChudLevel &top = lv[CHUD_THREAD_LEVELS];
+    // nodes of the last device level
+    const ChudLevel &top = lv[CHUD_DEVICE_LEVELS];
     std::vector<uint32_t> lens(3 * (size_t)top.count);
     HIP_TRY(hipMemcpyAsync(lens.data(), top.lp, lens.size() * 4, hipMemcpyDeviceToHost, C.stream));
     if ((st = C.sync())) return st;
@@ -619,13 +793,16 @@ namespace {
 size_t bits_of_top(uint64_t top, size_t limbs) { return limbs ? (limbs - 1) * 64 + 64 - (size_t)__builtin_clzll(top) : 0; }
 }  // namespace
 
-int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
+int to_decimal(const uint64_t *x, size_t xn, std::string &out, size_t point) {
     Context &C = ctx();
     out.clear();
     int st;
     size_t xb = 0;
     if ((st = top_len(x, xn, &xn))) return st;
-    if (!xn) return PG_OK;
+    if (!xn) {
+        if (point) out = "0." + std::string(point, '0');
+        return PG_OK;
+    }
     if ((st = bit_length(x, xn, &xb))) return st;
     // chunks of 19 digits: 2^L >= ceil(digits / 19) with digits <= bits * log10(2) + 1
     const size_t maxdig = (size_t)((double)xb * 0.30103) + 2;
@@ -783,8 +960,10 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
                        (const DecLeaf *)dl.d, (uint32_t)leaves.size(), v, (char *)chars.d, dcap);
     C.stage_end("decimal_leaf", e0, (double)dcap);
     HIP_TRY(hipGetLastError());
-    std::string s(dcap, '0');
-    HIP_TRY(hipMemcpyAsync(&s[0], chars.d, dcap, hipMemcpyDeviceToHost, C.stream));
+    // the string into page-locked memory (one DMA; the image is recycled for
+    // the next conversion), then into `out` in one copy
+    HostImg img = C.take_img(dcap);
+    HIP_TRY(hipMemcpyAsync(img.data(), chars.d, dcap, hipMemcpyDeviceToHost, C.stream));
     std::vector<std::vector<uint64_t>> fh(checks.size());
     for (size_t c = 0; c < checks.size(); ++c) {
         fh[c].resize(4 * checks[c].n);
@@ -794,8 +973,33 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out) {
     for (const auto &f : fh)
         for (size_t j = 0; j < f.size(); j += 4)
             if (f[j] != 1 || f[j + 2] != 0) return set_error(PG_EINTERNAL, "decimal split check");
-    const size_t first = s.find_first_not_of('0');
-    out = first == std::string::npos ? std::string() : s.substr(first);
+    const char *p = img.data();
+    size_t first = 0;
+    while (first < dcap && p[first] == '0') ++first;
+    const char *sig = p + first;
+    size_t n = dcap - first;
+    if (!point) {
+        out.assign(sig, n);
+    } else {
+        if (!n) {   // zero: "0" before the padding
+            sig = "0";
+            n = 1;
+        }
+        const size_t total = std::max(n, point + 1), zeros = total - n;
+        out.clear();
+        out.reserve(total + 1);
+        if (zeros) {   // integer part "0", then zeros - 1 zeros and the digits
+            out.push_back('0');
+            out.push_back('.');
+            out.append(zeros - 1, '0');
+            out.append(sig, n);
+        } else {
+            out.append(sig, n - point);
+            out.push_back('.');
+            out.append(sig + (n - point), point);
+        }
+    }
+    C.hold(std::move(img));
     return PG_OK;
 }
 
@@ -1009,8 +1213,7 @@ int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {
     }
     if ((st = ctx().sync())) return st;
     auto t3 = now();
-    std::string str;
-    if ((st = to_decimal(x.d, xl, str))) return st;
+    if ((st = to_decimal(x.d, xl, out, digits))) return st;
     auto t4 = now();
     if (phase_ms) {
         phase_ms[0] = ms(t0, t1);
@@ -1018,14 +1221,7 @@ int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {
         phase_ms[2] = ms(t2, t3);
         phase_ms[3] = ms(t3, t4);
     }
-    if (str.empty()) str = "0";   // to_dec, pi_chudnovsky.cpp:154
-    if (digits == 0) {
-        out = str;
-        return PG_OK;
-    }
-    if (str.size() <= digits) str.insert(0, digits + 1 - str.size(), '0');   // 190-191
-    str.insert(str.end() - (long long)digits, '.');
-    out.swap(str);
+    if (out.empty()) out = "0";   // to_dec, pi_chudnovsky.cpp:154 (digits == 0)
     return PG_OK;
 }
 

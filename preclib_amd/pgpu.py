@@ -356,7 +356,7 @@ def to_decimal(a):
     buf = ctypes.create_string_buffer(cap)
     n = SZ(0)
     _check(_lib.pg_to_decimal(buf, cap, ctypes.byref(n), _ptr(a), len(a)))
-    return buf.raw[:n.value].decode()
+    return ctypes.string_at(buf, n.value).decode("ascii")   # the n bytes only (buf.raw copies the whole buffer)
 
 
 def chud_split(nterms, need_p=True):
@@ -380,7 +380,7 @@ def pi(digits, want_phases=False):
     n = SZ(0)
     ph = (ctypes.c_double * 4)()
     _check(_lib.pg_pi(buf, cap, ctypes.byref(n), digits, ph))
-    s = buf.raw[:n.value].decode()
+    s = ctypes.string_at(buf, n.value).decode("ascii")
     return (s, list(ph)) if want_phases else s
 
 
