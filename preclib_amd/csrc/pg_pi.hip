@@ -28,8 +28,8 @@ namespace pg {
 
 constexpr uint64_t CHUD_C3_OVER_24 = 10939058860032000ull;   // pi_chudnovsky.cpp:11
 constexpr uint64_t CHUD_A = 13591409u, CHUD_B = 545140134u;   // 12-13
-constexpr int CHUD_THREAD_LEVELS = 4;                         // per-thread levels 1..4
-constexpr int CHUD_DEVICE_LEVELS = 8;                         // one wave per merge: levels 5..8
+constexpr int CHUD_THREAD_LEVELS = 2;                         // per-thread levels 1..2
+constexpr int CHUD_DEVICE_LEVELS = 8;                         // one wave per merge: levels 3..8
 constexpr uint64_t TEN19 = 10000000000000000000ull;
 constexpr int DEC_LEAF = 32;                                  // chunks per conversion thread
 
@@ -171,140 +171,107 @@ __global__ void __launch_bounds__(256) k_chud_merge(ChudLevel in, ChudLevel out)
 
 // ---- one wave per merge (levels CHUD_THREAD_LEVELS + 1 .. CHUD_DEVICE_LEVELS) ----------
 //
-// A level-j node has at most 3 * 2^j limbs (slot): one thread per merge left
-// most of the device idle from level 5 on (88 workgroups), and host-planned
-// batches of tens of thousands of small products cost more on the host than
-// on the GPU.  Here a wave forms each product as column sums: lane l owns
-// column c = 64 k + l of chunk k and sums the 128-bit products a_i b_(c-i)
-// in three words; one lane then turns the columns into limbs.
+// A level-j node has at most 3 * 2^j limbs (slot): one thread per merge took
+// 350 / 579 / 1911 us for levels 3 / 4 / 5 at 10^7 digits (level 5 on 88
+// workgroups), and host-planned batches of tens of thousands of small products
+// cost more on the host than on the GPU.  Here a wave forms each product 64
+// limbs at a time from column sums (wave_product): levels 3..8 take 159, 159,
+// 131, 208, 345 and 645 us.
 
-// 192-bit s += (x0, x1, x2)
-__device__ __forceinline__ void add192(uint64_t *s, uint64_t x0, uint64_t x1, uint64_t x2) {
-    const uint64_t y0 = s[0] + x0;
-    const uint64_t c0 = y0 < x0;
-    uint64_t y1 = s[1] + x1;
-    uint64_t c1 = y1 < x1;
-    y1 += c0;
-    c1 += y1 < c0;
-    s[0] = y0;
-    s[1] = y1;
-    s[2] += x2 + c1;
+// Value of lane (this lane - d) (d = 1, 2); lanes below d read 0.
+__device__ __forceinline__ uint64_t lane_down(uint64_t v, uint32_t lane, uint32_t d) {
+    const int src = (int)(lane >= d ? lane - d : 0) << 2;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(v >> 32));
+    return lane >= d ? ((uint64_t)hi << 32 | lo) : 0;
 }
 
-// Column sums of a * b (a, b in LDS, na, nb >= 1) added into sums[3 c ..
-// 3 c + 2], c < na + nb - 1.
-__device__ void wave_cols(uint64_t *sums, const uint64_t *a, uint32_t na, const uint64_t *b, uint32_t nb,
-                          uint32_t lane) {
-    const uint32_t ncols = na + nb - 1;
-    for (uint32_t c0 = 0; c0 < ncols; c0 += 64) {
-        const uint32_t c = c0 + lane;
-        uint64_t s0 = 0, s1 = 0, s2 = 0;
-        // the i some lane of the chunk uses: c - (nb - 1) <= i <= c
-        const uint32_t ilo = c0 + 1 > nb ? c0 + 1 - nb : 0;
-        const uint32_t ihi = min(na - 1, c0 + 63);
-        for (uint32_t i = ilo; i <= ihi; ++i) {
-            const uint32_t j = c - i;   // (wraps when i > c: then j >= nb)
-            if (i <= c && j < nb) {
-                const uint64_t x = a[i], y = b[j];
-                const uint64_t lo = x * y, hi = __umul64hi(x, y);
-                s0 += lo;
-                const uint64_t k0 = s0 < lo;
-                s1 += hi;
-                const uint64_t k1 = s1 < hi;
-                s1 += k0;
-                s2 += k1 + (s1 < k0);
-            }
+__device__ __forceinline__ uint64_t lane_at(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return (uint64_t)hi << 32 | lo;
+}
+
+// Adds the column sum of column c = c0 + lane of a * b (a global, read with
+// uniform indices; b in LDS; na, nb >= 1) into (s0, s1, s2): the 128-bit
+// products a_i b_(c-i).
+__device__ __forceinline__ void col_sum(uint64_t &s0, uint64_t &s1, uint64_t &s2, const uint64_t *a, uint32_t na,
+                                        const uint64_t *b, uint32_t nb, uint32_t c0, uint32_t lane) {
+    const uint32_t c = c0 + lane;
+    // the i some lane of the chunk uses: c - (nb - 1) <= i <= c
+    const uint32_t ilo = c0 + 1 > nb ? c0 + 1 - nb : 0;
+    const uint32_t ihi = min(na - 1, c0 + 63);
+#pragma unroll 4
+    for (uint32_t i = ilo; i <= ihi; ++i) {
+        const uint32_t j = c - i;   // (wraps when i > c: then j >= nb)
+        const uint64_t x = a[i];
+        if (i <= c && j < nb) {
+            const uint64_t y = b[j];
+            const uint64_t lo = x * y, hi = __umul64hi(x, y);
+            s0 += lo;
+            const uint64_t k0 = s0 < lo;
+            s1 += hi;
+            const uint64_t k1 = s1 < hi;
+            s1 += k0;
+            s2 += k1 + (s1 < k0);
         }
-        if (c < ncols) add192(sums + 3 * (size_t)c, s0, s1, s2);
     }
 }
 
-// 64-bit value of lane (this lane - 1); lane 0 reads 0.
-__device__ __forceinline__ uint64_t lane_below(uint64_t v, uint32_t lane) {
-    const int src = (int)(lane ? lane - 1 : 0) << 2;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)v);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)(v >> 32));
-    return lane ? ((uint64_t)hi << 32 | lo) : 0;
-}
-
-// dst[0, n) = sum over c < ncols of sums[c] 2^(64 c) (the value fits n
-// limbs, n >= ncols); returns the trimmed length.  64 limbs at a time: limb c
-// gathers w0 of column c, w1 of column c - 1 and w2 of column c - 2 as
-// lo + 2^64 e (e <= 2), takes e of the limb below, x = lo + e_below =
-// t + 2^64 g, and the carries into the 64 limbs follow from the generate (g)
-// and propagate (t = 2^64 - 1) masks by one integer addition: with
-// A = G | P and B = G, the carry into bit i of A + B + cin is bit i of
+// out[0, n) = x * y (+ u * v when un, vn > 0), the value below 2^(64 n);
+// returns the trimmed length.  y and v are staged in LDS (B, B2; slot words
+// each).  64 limbs at a time: lane l forms the column sum (s0, s1, s2) of
+// column c = c0 + l (complete within the chunk), limb c gathers s0 of column
+// c, s1 of column c - 1 and s2 of column c - 2 (from the lanes below, or the
+// previous chunk's top lanes) as lo + 2^64 e (e <= 2), adds e of the limb
+// below: x = t + 2^64 g; the carries into the chunk's limbs follow from the
+// generate (g) and propagate (t = 2^64 - 1) masks by one integer addition:
+// with A = G | P and B = G, the carry into bit i of A + B + cin is bit i of
 // (A + B + cin) ^ P.
-__device__ uint32_t wave_norm(uint64_t *dst, uint32_t n, const uint64_t *sums, uint32_t ncols, uint32_t lane) {
-    uint64_t e_in = 0;   // e of the limb below the chunk
-    uint64_t cin = 0;    // carry into the chunk's first limb
+__device__ uint32_t wave_product(uint64_t *out, uint32_t n, const uint64_t *x, uint32_t xn, const uint64_t *y,
+                                 uint32_t yn, const uint64_t *u, uint32_t un, const uint64_t *v, uint32_t vn,
+                                 uint64_t *B, uint64_t *B2, uint32_t lane) {
+    const bool p1 = xn && yn, p2 = un && vn;
+    for (uint32_t k = lane; k < (p1 ? yn : 0); k += 64) B[k] = y[k];
+    for (uint32_t k = lane; k < (p2 ? vn : 0); k += 64) B2[k] = v[k];
+    __syncthreads();
+    uint64_t s1_63 = 0, s2_62 = 0, s2_63 = 0;   // the previous chunk's top columns
+    uint64_t e_in = 0, cin = 0;                  // e of the limb below the chunk, carry into it
     uint32_t top = 0;
     for (uint32_t c0 = 0; c0 < n; c0 += 64) {
         const uint32_t c = c0 + lane;
-        uint64_t lo = 0, e = 0;
-        if (c < n) {
-            const uint64_t a = c < ncols ? sums[3 * (size_t)c] : 0;
-            const uint64_t b = (c >= 1 && c - 1 < ncols) ? sums[3 * (size_t)(c - 1) + 1] : 0;
-            const uint64_t d = (c >= 2 && c - 2 < ncols) ? sums[3 * (size_t)(c - 2) + 2] : 0;
-            lo = a + b;
-            e = lo < b;
-            lo += d;
-            e += lo < d;
-        }
-        const uint64_t below = lane_below(e, lane);
-        const uint64_t eb = lane ? below : e_in;
+        uint64_t s0 = 0, s1 = 0, s2 = 0;
+        if (p1) col_sum(s0, s1, s2, x, xn, B, yn, c0, lane);
+        if (p2) col_sum(s0, s1, s2, u, un, B2, vn, c0, lane);
+        const uint64_t d1 = lane_down(s1, lane, 1), d2 = lane_down(s2, lane, 2);
+        const uint64_t w1 = lane ? d1 : s1_63;
+        const uint64_t w2 = lane >= 2 ? d2 : (lane ? s2_63 : s2_62);
+        uint64_t lo = s0 + w1;
+        uint64_t e = lo < w1;
+        lo += w2;
+        e += lo < w2;
+        const uint64_t de = lane_down(e, lane, 1);   // (every lane takes part in the permute)
+        const uint64_t eb = lane ? de : e_in;
         const uint64_t t = lo + eb;
         const uint64_t G = __ballot(t < eb), Pm = __ballot(t == ~0ull);
         const uint64_t Cv = (((G | Pm) + G) + cin) ^ Pm;
         const uint64_t r = t + ((Cv >> lane) & 1);
-        if (c < n) dst[c] = r;
+        if (c < n) out[c] = r;
         const uint64_t nz = __ballot(c < n && r != 0);
         if (nz) top = c0 + 64 - (uint32_t)__clzll((long long)nz);
         cin = ((G >> 63) & 1) | ((Pm >> 63) & (Cv >> 63) & 1);
-        e_in = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e, 63);
+        e_in = lane_at(e, 63);
+        s1_63 = lane_at(s1, 63);
+        s2_62 = lane_at(s2, 62);
+        s2_63 = lane_at(s2, 63);
     }
+    __syncthreads();   // B, B2 are restaged by the next product
     return top;
-}
-
-// Copies n words global -> LDS by the wave.
-__device__ __forceinline__ void wave_stage(uint64_t *dst, const uint64_t *src, uint32_t n, uint32_t lane) {
-    for (uint32_t k = lane; k < n; k += 64) dst[k] = src[k];
-}
-
-// Zeroes n words of LDS.
-__device__ __forceinline__ void wave_zero(uint64_t *dst, uint32_t n, uint32_t lane) {
-    for (uint32_t k = lane; k < n; k += 64) dst[k] = 0;
-}
-
-// out = x * y (+ u * v when u is set) into n limbs; returns the trimmed length.
-// LDS: A, B (slot words each), S (3 (2 slot + 1) words).
-__device__ uint32_t wave_product(uint64_t *out, uint32_t n, const uint64_t *x, uint32_t xn, const uint64_t *y,
-                                 uint32_t yn, const uint64_t *u, uint32_t un, const uint64_t *v, uint32_t vn,
-                                 uint64_t *A, uint64_t *B, uint64_t *S, uint32_t lane) {
-    wave_zero(S, 3 * n, lane);
-    if (xn && yn) {
-        wave_stage(A, x, xn, lane);
-        wave_stage(B, y, yn, lane);
-        __syncthreads();
-        wave_cols(S, A, xn, B, yn, lane);
-        __syncthreads();
-    }
-    if (u && un && vn) {
-        wave_stage(A, u, un, lane);
-        wave_stage(B, v, vn, lane);
-        __syncthreads();
-        wave_cols(S, A, un, B, vn, lane);
-    }
-    __syncthreads();
-    const uint32_t len = wave_norm(out, n, S, n, lane);
-    __syncthreads();   // S and A, B are reused by the next product
-    return len;
 }
 
 // Level j -> j + 1 with one 64-thread workgroup per merge (the arithmetic of
 // k_chud_merge): Q = lq rq, P = lp rp, T = lt rq + rt lp; an unpaired last
-// node moves up unchanged.  Dynamic LDS: (2 slot + 3 (2 slot + 1)) words of
-// the input level.
+// node moves up unchanged.  Dynamic LDS: 2 slot words of the input level.
 __global__ void __launch_bounds__(64) k_chud_merge_wave(ChudLevel in, ChudLevel out) {
     extern __shared__ uint64_t sh[];
     const uint32_t i = blockIdx.x, lane = threadIdx.x;
@@ -326,11 +293,11 @@ __global__ void __launch_bounds__(64) k_chud_merge_wave(ChudLevel in, ChudLevel 
     }
     const uint64_t *rP = in.P + (size_t)r * in.slot, *rQ = in.Q + (size_t)r * in.slot, *rT = in.T + (size_t)r * in.slot;
     const uint32_t rlp = in.lp[r], rlq = in.lq[r], rlt = in.lt[r];
-    uint64_t *A = sh, *B = sh + in.slot, *S = sh + 2 * (size_t)in.slot;
-    const uint32_t q = wave_product(Q, llq + rlq, lQ, llq, rQ, rlq, nullptr, 0, nullptr, 0, A, B, S, lane);
-    const uint32_t p = wave_product(P, llp + rlp, lP, llp, rP, rlp, nullptr, 0, nullptr, 0, A, B, S, lane);
+    uint64_t *B = sh, *B2 = sh + in.slot;
+    const uint32_t q = wave_product(Q, llq + rlq, lQ, llq, rQ, rlq, nullptr, 0, nullptr, 0, B, B2, lane);
+    const uint32_t p = wave_product(P, llp + rlp, lP, llp, rP, rlp, nullptr, 0, nullptr, 0, B, B2, lane);
     const uint32_t tn = (llt + rlq > rlt + llp ? llt + rlq : rlt + llp) + 1;
-    const uint32_t t = wave_product(T, tn, lT, llt, rQ, rlq, rT, rlt, lP, llp, A, B, S, lane);
+    const uint32_t t = wave_product(T, tn, lT, llt, rQ, rlq, rT, rlt, lP, llp, B, B2, lane);
     if (lane == 0) {
         out.lp[i] = p;
         out.lq[i] = q;
@@ -420,9 +387,8 @@ int chud_split(size_t nterms, DBuf &Pout, size_t &lp, DBuf &Qout, size_t &lq, DB
         else if (j <= CHUD_THREAD_LEVELS)
             hipLaunchKernelGGL(k_chud_merge, dim3(blocks_for(count)), dim3(256), 0, C.stream, lv[j - 1], L);
         else
-            hipLaunchKernelGGL(k_chud_merge_wave, dim3((unsigned)count), dim3(64),
-                               (2 * (size_t)lv[j - 1].slot + 3 * (2 * (size_t)lv[j - 1].slot + 1)) * 8, C.stream,
-                               lv[j - 1], L);
+            hipLaunchKernelGGL(k_chud_merge_wave, dim3((unsigned)count), dim3(64), 2 * (size_t)lv[j - 1].slot * 8,
+                               C.stream, lv[j - 1], L);
         C.stage_end(j <= CHUD_THREAD_LEVELS ? "chud_thread" : "chud_wave", e0, 0.0);
         HIP_TRY(hipGetLastError());
         if (count == 1 && j < CHUD_DEVICE_LEVELS) {   // done early: carry the root up

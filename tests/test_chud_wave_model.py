@@ -1,11 +1,12 @@
 """CPU model of the wave products of the pi binary splitting
-(pg_pi.hip: wave_cols, wave_norm, wave_product; k_chud_merge_wave).
+(pg_pi.hip: col_sum, wave_product; k_chud_merge_wave).
 
-wave_cols: lane l of chunk k sums the 128-bit products a_i b_(c-i) of column
-c = 64 k + l in three words; wave_norm turns the column sums into limbs 64 at
-a time: limb c gathers w0 of column c, w1 of column c - 1 and w2 of column
-c - 2 as lo + 2^64 e, adds e of the limb below (x = t + 2^64 g), and the
-carries into the chunk's limbs come from the ballot masks G (generate) and P
+col_sum: lane l of chunk k sums the 128-bit products a_i b_(c-i) of column
+c = 64 k + l in three words (s0, s1, s2); wave_product then makes the
+chunk's 64 limbs: limb c gathers s0 of column c, s1 of column c - 1 and s2 of
+column c - 2 (lanes below, or the previous chunk's lanes 62 and 63) as
+lo + 2^64 e, adds e of the limb below (x = t + 2^64 g), and the carries into
+the chunk's limbs come from the ballot masks G (generate) and P
 (t = 2^64 - 1) by one integer addition: carry into bit i = bit i of
 ((G | P) + G + cin) ^ P, the chunk's carry-out g_63 | (p_63 & c_63).  The
 model runs those steps on Python integers and is compared with the exact
@@ -25,51 +26,52 @@ def to_int(limbs):
     return sum(v << (64 * i) for i, v in enumerate(limbs))
 
 
-def wave_cols(sums, a, b):
+def col_sum(a, b, c0, lane):
+    """col_sum: the column sum of column c0 + lane of a * b (three words)."""
     na, nb = len(a), len(b)
-    ncols = na + nb - 1
-    for c0 in range(0, ncols, 64):
-        ilo = c0 + 1 - nb if c0 + 1 > nb else 0
-        ihi = min(na - 1, c0 + 63)
-        for lane in range(64):
-            c = c0 + lane
-            s = 0
-            for i in range(ilo, ihi + 1):
-                j = c - i
-                if i <= c and j < nb:
-                    s += a[i] * b[j]
-            assert s < 1 << 192
-            if c < ncols:
-                w = sums[c] + s
-                sums[c] = w & ((1 << 192) - 1)   # add192 (the sums stay below 2^192)
-                assert w < 1 << 192
+    c = c0 + lane
+    ilo = c0 + 1 - nb if c0 + 1 > nb else 0
+    ihi = min(na - 1, c0 + 63)
+    s = 0
+    for i in range(ilo, ihi + 1):
+        j = c - i
+        if i <= c and j < nb:
+            s += a[i] * b[j]
+    assert s < 1 << 192
+    return s
 
 
-def wave_norm(sums, ncols, n, formula=None):
+def wave_product(n, x, y, u=None, v=None, formula=None):
+    """wave_product: 64 limbs at a time from the chunk's column sums."""
     formula = formula or (lambda G, P, cin: ((((G | P) + G) & M64) + cin) & M64 ^ P)
     out = [0] * n
-    e_in, cin = 0, 0
+    s1_63 = s2_62 = s2_63 = 0
+    e_in = cin = 0
     for c0 in range(0, n, 64):
+        S = []
+        for lane in range(64):
+            s = 0
+            if x and y:
+                s += col_sum(x, y, c0, lane)
+            if u and v:
+                s += col_sum(u, v, c0, lane)
+            assert s < 1 << 192
+            S.append((s & M64, (s >> 64) & M64, s >> 128))
         los, es = [], []
         for lane in range(64):
-            c = c0 + lane
-            lo = e = 0
-            if c < n:
-                a = sums[c] & M64 if c < ncols else 0
-                b = (sums[c - 1] >> 64) & M64 if c >= 1 and c - 1 < ncols else 0
-                d = sums[c - 2] >> 128 if c >= 2 and c - 2 < ncols else 0
-                s = a + b + d
-                lo, e = s & M64, s >> 64
-                assert e <= 2
-            los.append(lo)
-            es.append(e)
+            w1 = S[lane - 1][1] if lane else s1_63
+            w2 = S[lane - 2][2] if lane >= 2 else (s2_63 if lane else s2_62)
+            t = S[lane][0] + w1 + w2
+            los.append(t & M64)
+            es.append(t >> 64)
+            assert es[-1] <= 2
         ts, G, Pm = [], 0, 0
         for lane in range(64):
             eb = es[lane - 1] if lane else e_in
-            x = los[lane] + eb
-            t = x & M64
+            xx = los[lane] + eb
+            t = xx & M64
             ts.append(t)
-            G |= (x >> 64) << lane
+            G |= (xx >> 64) << lane
             Pm |= (t == M64) << lane
         assert G & Pm == 0
         Cv = formula(G, Pm, cin)
@@ -79,17 +81,9 @@ def wave_norm(sums, ncols, n, formula=None):
                 out[c] = (ts[lane] + ((Cv >> lane) & 1)) & M64
         cin = ((G >> 63) & 1) | ((Pm >> 63) & (Cv >> 63) & 1)
         e_in = es[63]
-    assert cin == 0 and e_in == 0   # the value fits n limbs
+        s1_63, s2_62, s2_63 = S[63][1], S[62][2], S[63][2]
+    assert cin == 0 and e_in == 0 and s1_63 == 0 and s2_62 == 0 and s2_63 == 0   # the value fits n limbs
     return out
-
-
-def wave_product(n, x, y, u=None, v=None, formula=None):
-    sums = [0] * n
-    if x and y:
-        wave_cols(sums, x, y)
-    if u and v:
-        wave_cols(sums, u, v)
-    return wave_norm(sums, n, n, formula)
 
 
 def rand_limbs(rng, k, kind):
