@@ -267,6 +267,33 @@ __device__ __forceinline__ uint64_t lane63(uint64_t v) {
     return (uint64_t)hi << 32 | lo;
 }
 
+// t_j, g_j, p_j of limbs j = base + 64 it (lane l of the wave at base0 + l)
+// from their sums (lo, hi; 0 for j outside [0, m)): the high word of limb
+// j - 1 comes from the lane below (lane 0: lane 63 of the previous iteration,
+// or for the first, `below`, the high word of limb base0 - 1).
+template <int IT>
+__device__ __forceinline__ void lanes_t(const uint64_t (&lo)[IT], const uint64_t (&hi)[IT], uint64_t below, long base,
+                                        long m, uint64_t (&t)[IT], uint32_t (&g)[IT], uint32_t (&p)[IT]) {
+    const int lane = threadIdx.x & 63;
+    uint64_t hprev_last = below;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const long j = base + 64 * it;
+        const uint64_t up = lane_up1(hi[it]);
+        const uint64_t hprev = lane ? up : hprev_last;   // limb j - 1 (0 beyond m)
+        hprev_last = lane63(hi[it]);
+        if (j < m) {
+            t[it] = lo[it] + hprev;
+            g[it] = t[it] < lo[it];
+            p[it] = t[it] == ~0ull;
+        } else {
+            t[it] = j == m ? hprev : 0;
+            g[it] = 0;
+            p[it] = 0;
+        }
+    }
+}
+
 // Limbs t_j, j = base + 64 it (base >= 0, lane l of a wave at base0 + l), of
 // a blocked node.  Every lane loads both layers' words at its j (indices
 // clamped into [0, m), words no block covers discarded after), so the loads
@@ -326,13 +353,113 @@ __device__ __forceinline__ void limbs_t_blocked(const CarryNode &N, long base, u
     }
 }
 
+// The two words shifted_limb(v, n, sh, j) combines (a: the word shifted
+// toward bit 0's side, b: its neighbour), as indices into v; -1 where the
+// word lies outside [0, n).
+__device__ __forceinline__ void shift_words(long n, int64_t sh, long j, long &ia, long &ib) {
+    if (sh >= 0) {
+        ia = j + (long)((uint64_t)sh >> 6);
+        ib = ia + 1;
+    } else {
+        ia = j - (long)((uint64_t)(-sh) >> 6);
+        ib = ia - 1;
+    }
+    if (ia < 0 || ia >= n) ia = -1;
+    if (ib < 0 || ib >= n) ib = -1;
+}
+
+__device__ __forceinline__ uint64_t shift_combine(uint64_t a, uint64_t b, int64_t sh) {
+    if (sh >= 0) {
+        const int r = (int)(sh & 63);
+        return r ? (a >> r) | (b << (64 - r)) : a;
+    }
+    const int r = (int)((-sh) & 63);
+    return r ? (a << r) | (b >> (64 - r)) : a;
+}
+
+// shifted_limb(v, n, sh, j) for limbs j = base + 64 it: both words of every
+// iteration loaded before any is used (indices clamped, words outside
+// [0, n) read as 0), then combined.
+template <int IT>
+__device__ __forceinline__ void shifted_limbs(const uint64_t *vp, long n, int64_t sh, long base, uint64_t (&w)[IT]) {
+    if (!vp || !n) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) w[it] = 0;
+        return;
+    }
+    const auto v = gptr(vp);
+    uint64_t a[IT], b[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        long ia, ib;
+        shift_words(n, sh, base + 64 * it, ia, ib);
+        a[it] = v[ia < 0 ? 0 : ia];
+        b[it] = v[ib < 0 ? 0 : ib];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        long ia, ib;
+        shift_words(n, sh, base + 64 * it, ia, ib);
+        w[it] = shift_combine(ia < 0 ? 0 : a[it], ib < 0 ? 0 : b[it], sh);
+    }
+}
+
+// Limbs t_j, j = base + 64 it (base >= 0, lane l of a wave at base0 + l), of
+// a linear node (limb_sum's arithmetic): both operands' words for every
+// iteration are loaded before any is used.
+template <int IT>
+__device__ __forceinline__ void limbs_t_lin(const CarryNode &N, long base, uint64_t (&t)[IT], uint32_t (&g)[IT],
+                                            uint32_t (&p)[IT]) {
+    const long m = (long)N.m;
+    const int lane = threadIdx.x & 63;
+    const bool ycond = lin_cond(N);   // (with y null the term is 0, or ~0 when yneg)
+    uint64_t xv[IT], yv[IT];
+    shifted_limbs<IT>(N.x, (long)N.xn, N.xsh, base, xv);
+    shifted_limbs<IT>(ycond ? N.y : nullptr, (long)N.yn, N.ysh, base, yv);
+    uint64_t below = 0;   // high word of the limb below the wave's first
+    {
+        const long jb = base - lane - 1;
+        if (jb >= 0 && jb < m) {
+            uint64_t lo;
+            limb_sum(N, (size_t)jb, lo, below);
+        }
+    }
+    uint64_t lo[IT], hi[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const long j = base + 64 * it;
+        lo[it] = 0;
+        hi[it] = 0;
+        if (j < m) {
+            uint64_t l = xv[it] * N.xmul, h = __umul64hi(xv[it], N.xmul);
+            if (N.pow2 >= 0 && j == (long)((uint64_t)N.pow2 >> 6)) {
+                const uint64_t bit = 1ull << (N.pow2 & 63);
+                l += bit;
+                h += l < bit;
+            }
+            if (ycond) {
+                const uint64_t y = N.yneg ? ~yv[it] : yv[it];
+                l += y;
+                h += l < y;
+            }
+            lo[it] = l;
+            hi[it] = h;
+        }
+    }
+    lanes_t<IT>(lo, hi, below, base, m, t, g, p);
+}
+
 // Limbs t_j, j = base + 64 it, of node N.  Launches with a node table take
-// the product form for plain whole-limb nodes (loads issued together) and the
-// stepped form for blocked ones; the node is uniform across the workgroup.
-template <bool PLAIN, int IT>
+// the product form for plain whole-limb nodes, the layer form for blocked ones
+// and the shifted-word form for linear ones (every load of a thread issued
+// before the first is used; blocked and linear nodes take limb j - 1's high
+// word from the lane below); the node is uniform across the workgroup.
+template <bool PLAIN, bool LIN, int IT>
 __device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint64_t (&t)[IT], uint32_t (&g)[IT],
                                             uint32_t (&p)[IT]) {
-    if (PLAIN || (N.kind == NODE_PLAIN && N.parts == 1)) {
+    if (LIN && base - (long)(threadIdx.x & 63) >= 0) {   // a launch of linear nodes only (uniform per wave)
+        limbs_t_lin<IT>(N, base, t, g, p);
+    } else if (PLAIN || (N.kind == NODE_PLAIN && N.parts == 1)) {
         limbs_t_plain<IT>(N, base, t, g, p);
     } else if (N.kind == NODE_BLOCKED && base - (long)(threadIdx.x & 63) >= 0) {   // (uniform per wave)
         limbs_t_blocked<IT>(N, base, t, g, p);
@@ -352,7 +479,7 @@ __device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint6
 }
 
 // PLAIN: C.one is a plain node with whole-limb sums and there is no node table.
-template <bool PLAIN>
+template <bool PLAIN, bool LIN = false>
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch C) {
     __shared__ Agg waves[CARRY_WAVES];
     __shared__ uint32_t wave_in[CARRY_WAVES];
@@ -367,11 +494,11 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch 
     uint64_t t[IT1];
     uint32_t g[IT1], p[IT1];
     if (wave == 0 && !start) {
-        chunk_limbs<PLAIN, IT1>(N, base - 64, t, g, p);
+        chunk_limbs<PLAIN, LIN, IT1>(N, base - 64, t, g, p);
     } else {
         uint64_t t0[CARRY_ITERS];
         uint32_t g0[CARRY_ITERS], p0[CARRY_ITERS];
-        chunk_limbs<PLAIN, CARRY_ITERS>(N, base, t0, g0, p0);
+        chunk_limbs<PLAIN, LIN, CARRY_ITERS>(N, base, t0, g0, p0);
         t[0] = 0; g[0] = 0; p[0] = 0;
 #pragma unroll
         for (int it = 0; it < CARRY_ITERS; ++it) {
@@ -460,7 +587,7 @@ __device__ uint32_t fix_carry_in(const CarryLaunch &C, size_t chunk) {
 // propagate (written as 2^64 - 1, now 0) into the first that does not
 // (written as t, now t + 1).  Limbs past out_len are not stored but still
 // carry the ripple; limb m is the flag word.
-template <bool PLAIN>
+template <bool PLAIN, bool LIN = false>
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C) {
     const size_t chunk = (size_t)blockIdx.x * CARRY_WAVES + (threadIdx.x >> 6);
     if (chunk >= C.nchunks) return;
@@ -473,7 +600,7 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C
         const long j = cbase + 64 * grp + lane;
         uint64_t t[1];
         uint32_t g[1], p[1];
-        chunk_limbs<PLAIN, 1>(N, j, t, g, p);
+        chunk_limbs<PLAIN, LIN, 1>(N, j, t, g, p);
         const uint64_t stop = __ballot(!p[0]);
         const int k = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
         if (lane <= k) {   // k == 64: the whole group propagates
@@ -495,6 +622,10 @@ hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream) {
         hipLaunchKernelGGL(k_carry_spec<true>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
         if (!one_pass)
             hipLaunchKernelGGL(k_carry_fix<true>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+    } else if (C.lin_only) {   // its own instance: the linear loader's registers stay out of the others
+        hipLaunchKernelGGL((k_carry_spec<false, true>), dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
+        if (!one_pass)
+            hipLaunchKernelGGL((k_carry_fix<false, true>), dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
     } else {
         hipLaunchKernelGGL(k_carry_spec<false>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
         if (!one_pass)

@@ -359,6 +359,7 @@ struct CarryLaunch {
     size_t nchunks;
     uint8_t *agg;               // per-chunk G | P << 1 | used carry-in << 2 | resolved << 3
     size_t nnodes;              // nodes of the launch (1 for `one`; 0: not stated)
+    int lin_only;               // every node is NODE_LIN (loaded by limbs_t_lin)
 };
 
 // Enqueues the carry of C on `stream`: the speculative pass and the fix-up

@@ -16,6 +16,10 @@ read plain whole-limb nodes with the product form's clamped loads, and
 blocked nodes with both layers' clamped loads at every lane's limb plus the
 limb below the wave.
 
+Linear nodes are read with both operands' word pairs at every lane's limb
+(shift_words; index 0 where a word lies outside the operand) plus the limb
+below the wave.
+
 The blocked layout itself is checked on values: block k of a blocked
 multiply (Batch::plan) is summed into layer k & 1 at limb k B, the blocks of
 a layer never overlap, and limb j's sum read through blocked_cover equals the
@@ -126,6 +130,36 @@ def blocked_wave_reads(N, base0, iters):
     return out
 
 
+def shift_words(n, sh, j):
+    """shift_words (pg_carry.hip): the indices of the two words shifted_limb
+    combines, -1 outside [0, n)."""
+    if sh >= 0:
+        ia = j + (sh >> 6)
+        ib = ia + 1
+    else:
+        ia = j - ((-sh) >> 6)
+        ib = ia - 1
+    return (ia if 0 <= ia < n else -1), (ib if 0 <= ib < n else -1)
+
+
+def lin_wave_reads(N, base0, iters):
+    """limbs_t_lin for the wave whose lane l starts at limb base0 + l: both
+    operands' two words at every lane and iteration (index 0 where the word
+    is outside the operand), and limb_sum of the limb below the wave."""
+    out = []
+    for it in range(iters):
+        for lane in range(64):
+            j = base0 + lane + 64 * it
+            for arr, n, sh in (("x", N.xn, N.xsh), ("y", N.yn, N.ysh)):
+                if n:
+                    ia, ib = shift_words(n, sh, j)
+                    out += [(arr, max(ia, 0)), (arr, max(ib, 0))]
+    jb = base0 - 1
+    if 0 <= jb < N.m:
+        out += limb_sum_reads(N, jb)
+    return out
+
+
 def t_reads(N, j, plain):
     """Reads of limbs_t_plain (the product launch, and plain whole-limb nodes
     of node-table launches) or limb_t for limb j of the window."""
@@ -164,8 +198,9 @@ def spec_accesses(nodes, cmap, plain, reads=t_reads):
             base = cbase + wave * 64 * ITERS + np.arange(64)
             window = wave == 0 and not start
             b0 = base - 64 if window else base
-            if N.kind == BLOCKED and not plain and reads is t_reads:
-                for a, i in blocked_wave_reads(N, int(b0[0]), ITERS + (1 if window else 0)):
+            if N.kind in (BLOCKED, LIN) and not plain and reads is t_reads:
+                wave_reads = blocked_wave_reads if N.kind == BLOCKED else lin_wave_reads
+                for a, i in wave_reads(N, int(b0[0]), ITERS + (1 if window else 0)):
                     check_node_access(N, a, i)
             else:
                 for it in range(ITERS + (1 if window else 0)):
@@ -201,12 +236,13 @@ def fix_accesses(nodes, cmap, agg_resolved, plain):
                     break
                 top -= 64
             cbase = (chunk - N.chunk0) * CHUNK
-            if N.kind == BLOCKED and not plain:
+            if N.kind in (BLOCKED, LIN) and not plain:
+                wave_reads = blocked_wave_reads if N.kind == BLOCKED else lin_wave_reads
                 for grp in range(CHUNK // 64):
-                    for a, i in blocked_wave_reads(N, cbase + 64 * grp, 1):
+                    for a, i in wave_reads(N, cbase + 64 * grp, 1):
                         check_node_access(N, a, i)
             for j in range(cbase, cbase + CHUNK):
-                reads = [] if (N.kind == BLOCKED and not plain) else t_reads(N, j, plain)
+                reads = [] if (N.kind in (BLOCKED, LIN) and not plain) else t_reads(N, j, plain)
                 for a, i in reads + writes(N, j):
                     check_node_access(N, a, i)
 
@@ -401,3 +437,87 @@ def test_blocked_loader_lanes(ln, sn):
             continue
         for j, tgp in blocked_wave(layers, N, base0, ITERS + 1):
             assert tgp == t_g_p(exact, j), (base0, j)
+
+
+def shifted_limb(v, sh, j):
+    """shifted_limb (pg_carry.hip) from the value: limb j of v >> sh (sh >= 0)
+    or v << -sh."""
+    x = to_int(v)
+    x = x >> sh if sh >= 0 else x << (-sh)
+    return (x >> (64 * j)) & M64 if j >= 0 else 0
+
+
+def to_int(limbs):
+    return sum(int(w) << (64 * i) for i, w in enumerate(limbs))
+
+
+def lin_sums(x, xsh, xmul, y, ysh, yneg, pow2, cond, m):
+    """Limb sums S_j = lo_j + 2^64 hi_j of a linear node, from its definition."""
+    S = []
+    for j in range(m):
+        s = shifted_limb(x, xsh, j) * xmul
+        if pow2 >= 0 and j == pow2 >> 6:
+            s += 1 << (pow2 & 63)
+        if cond:
+            yv = shifted_limb(y, ysh, j) if y else 0
+            s += (~yv & M64) if yneg else yv
+        S.append(s)
+    return S
+
+
+def lin_wave(x, xsh, xmul, y, ysh, yneg, pow2, cond, m, base0, iters):
+    """limbs_t_lin's lanes: words by shift_words / shift_combine, sums per
+    lane, then lanes_t (limb j - 1's high word from the lane below)."""
+    def words(v, sh, j):
+        n = len(v) if v else 0
+        ia, ib = shift_words(n, sh, j)
+        a = v[ia] if (n and ia >= 0) else 0
+        b = v[ib] if (n and ib >= 0) else 0
+        if sh >= 0:
+            r = sh & 63
+            return ((a >> r) | (b << (64 - r))) & M64 if r else a
+        r = (-sh) & 63
+        return ((a << r) | (b >> (64 - r))) & M64 if r else a
+    def limb(j):
+        if not (0 <= j < m):
+            return 0, 0
+        s = words(x, xsh, j) * xmul
+        if pow2 >= 0 and j == pow2 >> 6:
+            s += 1 << (pow2 & 63)
+        if cond:
+            yv = words(y, ysh, j)
+            s += (~yv & M64) if yneg else yv
+        return s & M64, s >> 64
+    below = limb(base0 - 1)[1]
+    last = below
+    out = []
+    for it in range(iters):
+        lo_hi = [limb(base0 + lane + 64 * it) for lane in range(64)]
+        for lane in range(64):
+            j = base0 + lane + 64 * it
+            hprev = lo_hi[lane - 1][1] if lane else last
+            if j < m:
+                s = lo_hi[lane][0] + hprev
+                out.append((j, (s & M64, s >> 64, int((s & M64) == M64))))
+            else:
+                out.append((j, (hprev if j == m else 0, 0, 0)))
+        last = lo_hi[63][1]
+    return out
+
+
+@pytest.mark.parametrize("xsh,ysh", [(0, 0), (1, 63), (64, -64), (130, -5), (-1, 200), (-130, 0), (5000, -70)])
+@pytest.mark.parametrize("yneg,cond,ynull", [(0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 0, 0), (1, 0, 0)])
+def test_lin_loader_lanes(xsh, ysh, yneg, cond, ynull):
+    """limbs_t_lin's shifted words and cross-lane high words give the t, g, p
+    of the node's definition xmul X' + 2^pow2 + (yneg ? ~Y' : Y') (the y term
+    only when the condition holds; y null reads as zero), up to limb m + 64."""
+    rng = random.Random(xsh * 7 + ysh + 100 * yneg + 10 * cond)
+    m = 300
+    x = [rng.choice([M64, 0, rng.getrandbits(64)]) for _ in range(rng.randrange(1, 400))]
+    y = None if ynull else [rng.choice([M64, 0, rng.getrandbits(64)]) for _ in range(rng.randrange(1, 400))]
+    xmul = rng.choice([1, 10005, (1 << 59) - 1])
+    pow2 = rng.choice([-1, 0, 63, 64 * 150 + 7])
+    S = lin_sums(x, xsh, xmul, y, ysh, yneg, pow2, cond, m)
+    for base0 in (0, 64, 1984, 256):
+        for j, tgp in lin_wave(x, xsh, xmul, y, ysh, yneg, pow2, cond, m, base0, ITERS + 1):
+            assert tgp == t_g_p(S, j), (base0, j)
