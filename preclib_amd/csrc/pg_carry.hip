@@ -586,16 +586,23 @@ __device__ uint32_t fix_carry_in(const CarryLaunch &C, size_t chunk) {
 // written with 0; add 1 at its lowest limb, rippling through the limbs that
 // propagate (written as 2^64 - 1, now 0) into the first that does not
 // (written as t, now t + 1).  Limbs past out_len are not stored but still
-// carry the ripple; limb m is the flag word.
+// carry the ripple; limb m is the flag word.  A chunk past out_len that does
+// not hold the flag word stores nothing and returns at once (the look-backs of
+// other chunks read only the first pass's aggregates): a difference y - p
+// whose top limbs cancel, computed for its flag over all of y but stored in
+// fewer limbs, otherwise walked back and rippled through every chunk of its
+// propagating top part.
 template <bool PLAIN, bool LIN = false>
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C) {
     const size_t chunk = (size_t)blockIdx.x * CARRY_WAVES + (threadIdx.x >> 6);
     if (chunk >= C.nchunks) return;
     const uint32_t v = C.agg[chunk];
-    if ((v & 8) || !fix_carry_in(C, chunk)) return;
+    if (v & 8) return;
     const CarryNode &N = PLAIN ? C.one : node_of(C, chunk);
-    const int lane = threadIdx.x & 63;
     const long cbase = (long)(chunk - N.chunk0) * CARRY_CHUNK;
+    if (cbase >= (long)N.out_len && !(N.flag && (long)N.m >= cbase && (long)N.m < cbase + CARRY_CHUNK)) return;
+    if (!fix_carry_in(C, chunk)) return;
+    const int lane = threadIdx.x & 63;
     for (int grp = 0; grp < CARRY_CHUNK / 64; ++grp) {
         const long j = cbase + 64 * grp + lane;
         uint64_t t[1];

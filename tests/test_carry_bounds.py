@@ -224,6 +224,9 @@ def fix_accesses(nodes, cmap, agg_resolved, plain):
             if chunk >= nch or agg_resolved[chunk]:
                 continue
             N = nodes[cmap[chunk]]
+            cbase = (chunk - N.chunk0) * CHUNK
+            if cbase >= N.out_len and not (N.flag and cbase <= N.m < cbase + CHUNK):
+                continue   # stores nothing: k_carry_fix returns before its look-back
             top = chunk - 1
             while True:
                 idx = [top - lane for lane in range(64)]

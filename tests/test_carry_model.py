@@ -16,9 +16,14 @@ CHUNK = 256
 WINDOW = 64
 
 
-def normalize(slo, shi, out_len):
+def normalize(slo, shi, out_len, flag=False):
+    """Limbs [0, out_len) of the normalized value; with flag, also the flag
+    word (limb m of the value, m = len(slo)), as (limbs, flag word).  The
+    fix-up skips a chunk past out_len that does not hold limb m, as
+    k_carry_fix does."""
     m = len(slo)
-    nch = (max(m, out_len) + CHUNK - 1) // CHUNK
+    nch = (max(m + (1 if flag else 0), out_len) + CHUNK - 1) // CHUNK
+    fw = [None]
 
     def tgp(j):
         hprev = shi[j - 1] if 1 <= j <= m else 0
@@ -42,6 +47,8 @@ def normalize(slo, shi, out_len):
             t, g, p = tgp(j)
             if j < out_len:
                 out[j] = (t + x) & M64
+            if j == m and flag:
+                fw[0] = (t + x) & M64
             x = g | (p & x)
             g_all, p_all = g | (p & g_all), p & p_all
         agg.append((g_all, p_all, cin, resolved))
@@ -59,15 +66,22 @@ def normalize(slo, shi, out_len):
             i -= 1
 
     for c in range(nch):
-        if agg[c][3] or not carry_in(c):
+        base = c * CHUNK
+        if agg[c][3]:
             continue
-        for j in range(c * CHUNK, (c + 1) * CHUNK):
+        if base >= out_len and not (flag and base <= m < base + CHUNK):
+            continue   # stores nothing (k_carry_fix returns at once)
+        if not carry_in(c):
+            continue
+        for j in range(base, base + CHUNK):
             t, _, p = tgp(j)
             if j < out_len:
                 out[j] = (t + 1) & M64
+            if j == m and flag:
+                fw[0] = (t + 1) & M64
             if not p:
                 break
-    return out
+    return (out, fw[0]) if flag else out
 
 
 def check(slo, shi, out_len):
@@ -112,3 +126,28 @@ def test_random_mixtures():
         slo = [M64 if rng.random() < 0.97 else rng.getrandbits(64) for _ in range(m)]
         shi = [1 if rng.random() < 0.01 else 0 for _ in range(m)]
         check(slo, shi, m + rng.choice([0, 1]))
+
+
+def test_difference_with_flag_stored_short():
+    """y - p as y + ~p + 1 over m limbs, stored in out_len << m limbs, with
+    the flag word (limb m: 1 when y >= p): the top limbs cancel, so every
+    chunk above the stored part propagates and the fix-up skips the chunks
+    that store nothing; the low limbs and the flag stay exact."""
+    rng = random.Random(13)
+    for m, out_len in ((1500, 300), (2049, 257), (1024, 1), (700, 699)):
+        for delta in (0, 1, 5, -1, -7):
+            p = rng.getrandbits(64 * (m - 2)) | 1 << (64 * (m - 2))
+            y = p + delta * (1 + rng.getrandbits(64 * (out_len // 2 + 1)) if delta else 0)
+            if y < 0:
+                continue
+            yl = [(y >> (64 * j)) & M64 for j in range(m)]
+            pl = [(~(p >> (64 * j))) & M64 for j in range(m)]
+            slo = [(a + b) & M64 for a, b in zip(yl, pl)]
+            shi = [(a + b) >> 64 for a, b in zip(yl, pl)]
+            # + 1 (two's complement) folded into limb 0's sum
+            s0 = slo[0] + 1
+            slo[0], shi[0] = s0 & M64, shi[0] + (s0 >> 64)
+            out, fw = normalize(slo, shi, out_len, flag=True)
+            exact = y + ((1 << (64 * m)) - 1 - p) + 1   # y + ~p + 1 over m limbs, carry-out at limb m
+            assert out == [(exact >> (64 * j)) & M64 for j in range(out_len)], (m, out_len, delta)
+            assert fw == exact >> (64 * m), (m, out_len, delta)
