@@ -32,6 +32,7 @@ constexpr int CHUD_THREAD_LEVELS = 2;                         // per-thread leve
 constexpr int CHUD_DEVICE_LEVELS = 8;                         // one wave per merge: levels 3..8
 constexpr uint64_t TEN19 = 10000000000000000000ull;
 constexpr int DEC_LEAF = 32;                                  // chunks per conversion thread
+constexpr size_t DEC_WAVE_MAX_LIMBS = 256;                    // split levels with W_i of at most this many limbs: one wave per division
 
 // ---- per-thread integers -----------------------------------------------------------
 
@@ -267,6 +268,89 @@ __device__ uint32_t wave_product(uint64_t *out, uint32_t n, const uint64_t *x, u
     }
     __syncthreads();   // B, B2 are restaged by the next product
     return top;
+}
+
+// dst[0, store) = the low limbs of a + (neg ? ~b : b) + cin over m limbs (a:
+// an limbs, b: bn limbs, zero beyond; dst may be null with store 0); returns
+// the carry out of limb m - 1.  64 limbs at a time, the carries from the
+// generate / propagate ballots as in wave_product.
+__device__ uint32_t wave_add(uint64_t *dst, uint32_t store, const uint64_t *a, uint32_t an, const uint64_t *b,
+                             uint32_t bn, bool neg, uint32_t cin, uint32_t m, uint32_t lane) {
+    uint64_t c = cin;
+    for (uint32_t c0 = 0; c0 < m; c0 += 64) {
+        const uint32_t k = c0 + lane;
+        const bool in = k < m;
+        const uint64_t x = (in && k < an) ? a[k] : 0;
+        uint64_t y = (in && k < bn) ? b[k] : 0;
+        if (neg && in) y = ~y;
+        const uint64_t s = x + y;
+        const uint64_t G = __ballot(in && s < x), Pm = __ballot(in && s == ~0ull);
+        const uint64_t Cv = (((G | Pm) + G) + c) ^ Pm;
+        if (in && k < store) dst[k] = s + ((Cv >> lane) & 1);
+        const uint32_t top = m - c0 < 64 ? m - c0 : 64;   // limbs of this chunk
+        c = top == 64 ? (((G >> 63) & 1) | ((Pm >> 63) & (Cv >> 63) & 1)) : ((Cv >> top) & 1);
+    }
+    return (uint32_t)c;
+}
+
+// One division of the decimal conversion's split (to_decimal): y = q W + r by
+// the level's reciprocal V = floor(2^sb / W) (sb = 2 bits(W)), the steps of the
+// batched levels in one wave, intermediates in LDS:
+//   ax = y V,  q = ax >> sb (qlen limbs),  p = q W,
+//   r = y - p over max(n, qlen + nw) + 1 limbs (flag r >= 0: the carry out),
+//   t = r - W (f = r >= W),  r2 = f ? t : r,  q2 = q + f,  check r2 < W.
+// flags[0] = (r >= 0), flags[1] = f, flags[2] = (r2 >= W); the host reads them
+// after the last level.
+struct DecSplitJob {
+    const uint64_t *y;
+    uint64_t *r2, *q2;   // children: nw and qlen limbs
+    uint64_t *flags;
+    uint32_t n, qlen;
+};
+
+// LDS words of k_dec_split_wave for a level (nmax, qmax: the largest n and
+// qlen of its jobs).
+inline size_t dec_split_lds_words(size_t nmax, size_t qmax, size_t nw, size_t vb) {
+    return 2 * std::max(nw, vb) + (nmax + vb) + qmax + (qmax + nw) + 2 * (nw + 1);
+}
+
+__global__ void __launch_bounds__(64) k_dec_split_wave(const DecSplitJob *jobs, uint32_t nj, const uint64_t *W,
+                                                       uint32_t nw, const uint64_t *V, uint32_t vb, uint32_t sb,
+                                                       uint32_t nmax, uint32_t qmax) {
+    extern __shared__ uint64_t sh[];
+    const uint32_t jb = blockIdx.x, lane = threadIdx.x;
+    if (jb >= nj) return;
+    const DecSplitJob J = jobs[jb];
+    const uint32_t bm = nw > vb ? nw : vb;
+    uint64_t *B = sh, *B2 = B + bm, *AX = B2 + bm, *Q = AX + (nmax + vb), *Pp = Q + qmax, *R = Pp + (qmax + nw),
+             *T = R + (nw + 1);
+    const uint32_t axn = J.n + vb;
+    wave_product(AX, axn, J.y, J.n, V, vb, nullptr, 0, nullptr, 0, B, B2, lane);
+    {
+        const uint32_t s0 = sb >> 6, r = sb & 63;
+        for (uint32_t k = lane; k < J.qlen; k += 64) {
+            const uint32_t i = k + s0;
+            const uint64_t lo = i < axn ? AX[i] : 0, hi = i + 1 < axn ? AX[i + 1] : 0;
+            Q[k] = r ? (lo >> r) | (hi << (64 - r)) : lo;
+        }
+    }
+    __syncthreads();
+    wave_product(Pp, J.qlen + nw, Q, J.qlen, W, nw, nullptr, 0, nullptr, 0, B, B2, lane);
+    const uint32_t m = (J.n > J.qlen + nw ? J.n : J.qlen + nw) + 1;
+    const uint32_t f0 = wave_add(R, nw + 1, J.y, J.n, Pp, J.qlen + nw, true, 1, m, lane);
+    __syncthreads();
+    const uint32_t f = wave_add(T, nw + 1, R, nw + 1, W, nw, true, 1, nw + 1, lane);
+    __syncthreads();
+    const uint64_t *R2 = f ? T : R;
+    for (uint32_t k = lane; k < nw; k += 64) J.r2[k] = R2[k];
+    wave_add(J.q2, J.qlen, Q, J.qlen, nullptr, 0, false, f, J.qlen, lane);
+    const uint32_t ge = wave_add(nullptr, 0, R2, nw + 1, W, nw, true, 1, nw + 1, lane);
+    if (lane == 0) {
+        J.flags[0] = f0;
+        J.flags[1] = f;
+        J.flags[2] = ge;
+        J.flags[3] = 0;
+    }
 }
 
 // Level j -> j + 1 with one 64-thread workgroup per merge (the arithmetic of
@@ -837,8 +921,50 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out, size_t point) {
             nodes.swap(next);
             continue;
         }
-        // per job: ax (n + vb), q (ql), p (ql + nw), r (nw + 1), t (nw + 1), r2 (nw + 1), q2 (ql)
         std::vector<size_t> qlen(nj), off(nj);
+        size_t nmax = 0, qmax = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            const DNode &y = nodes[jobs_of[j]];
+            qlen[j] = limbs_for_bits(y.bits + 2 - wb[i]) + 1;
+            nmax = std::max(nmax, y.n);
+            qmax = std::max(qmax, qlen[j]);
+        }
+        if (nw <= DEC_WAVE_MAX_LIMBS && dec_split_lds_words(nmax, qmax, nw, vb) * 8 <= 64 * 1024) {
+            // one wave per division (k_dec_split_wave): only the children r2 (nw
+            // limbs) and q2 (ql) are stored
+            size_t total = 0;
+            for (size_t j = 0; j < nj; ++j) {
+                off[j] = total;
+                total += nw + qlen[j];
+            }
+            keep.emplace_back(total);
+            DBuf fl(4 * nj), dj;
+            if (!keep.back().d || !fl.d) return set_error(PG_ENOMEM, "decimal conversion");
+            std::vector<DecSplitJob> jobs(nj);
+            for (size_t j = 0; j < nj; ++j) {
+                const DNode &y = nodes[jobs_of[j]];
+                uint64_t *r2 = keep.back().d + off[j], *q2 = r2 + nw;
+                jobs[j] = DecSplitJob{y.p, r2, q2, fl.d + 4 * j, (uint32_t)y.n, (uint32_t)qlen[j]};
+                const size_t qb = std::min(y.bits + 2 - wb[i], wb[i]);
+                next.push_back({r2, nw, wb[i], y.chunk0});
+                next.push_back({q2, std::min(qlen[j], limbs_for_bits(qb)), qb, y.chunk0 + ((size_t)1 << i)});
+            }
+            if ((st = upload(dj, jobs))) return st;
+            const size_t lds = dec_split_lds_words(nmax, qmax, nw, vb) * 8;
+            hipEvent_t e0 = nullptr;
+            C.stage_begin(e0);
+            hipLaunchKernelGGL(k_dec_split_wave, dim3((unsigned)nj), dim3(64), lds, C.stream, (const DecSplitJob *)dj.d,
+                               (uint32_t)nj, W[i].d, (uint32_t)nw, V[i].d, (uint32_t)vb, (uint32_t)sb, (uint32_t)nmax,
+                               (uint32_t)qmax);
+            C.stage_end("decimal_wave", e0, 0.0);
+            HIP_TRY(hipGetLastError());
+            checks.push_back({fl.d, nj});
+            keep.push_back(std::move(fl));
+            keep.push_back(std::move(dj));
+            nodes.swap(next);
+            continue;
+        }
+        // per job: ax (n + vb), q (ql), p (ql + nw), r (nw + 1), t (nw + 1), r2 (nw + 1), q2 (ql)
         size_t total = 0;
         for (size_t j = 0; j < nj; ++j) {
             const DNode &y = nodes[jobs_of[j]];

@@ -122,3 +122,87 @@ def test_carry_formula_slip_caught():
     with pytest.raises(AssertionError):
         got = wave_product(136, a, b, formula=slip)
         assert to_int(got) == to_int(a) * to_int(b)
+
+
+# ---- the decimal conversion's split, one wave per division (k_dec_split_wave) ----
+
+def wave_add(a, b, neg, cin, m):
+    """wave_add: (limbs of a + (neg ? ~b : b) + cin over m limbs, carry out of
+    limb m - 1), 64 limbs at a time with the ballot carries."""
+    out = []
+    c = cin
+    for c0 in range(0, m, 64):
+        s_l, G, P = [], 0, 0
+        for lane in range(64):
+            k = c0 + lane
+            inn = k < m
+            x = a[k] if inn and k < len(a) else 0
+            y = b[k] if inn and k < len(b) else 0
+            if neg and inn:
+                y = ~y & M64
+            s = x + y
+            s_l.append(s & M64)
+            G |= (inn and s >> 64) << lane
+            P |= (inn and (s & M64) == M64) << lane
+        Cv = ((((G | P) + G) & M64) + c) & M64 ^ P
+        for lane in range(64):
+            if c0 + lane < m:
+                out.append((s_l[lane] + ((Cv >> lane) & 1)) & M64)
+        top = min(m - c0, 64)
+        c = (((G >> 63) & 1) | ((P >> 63) & (Cv >> 63) & 1)) if top == 64 else (Cv >> top) & 1
+    return out, c
+
+
+def limbs(v, n):
+    return [(v >> (64 * k)) & M64 for k in range(n)]
+
+
+def dec_split(y, ybits, W, V, wb):
+    """k_dec_split_wave's steps on limbs: returns (r2, q2, flags)."""
+    nw = (W.bit_length() + 63) // 64
+    vb = (V.bit_length() + 63) // 64
+    n = max(1, (ybits + 63) // 64)
+    qlen = (ybits + 2 - wb + 63) // 64 + 1
+    sb = 2 * wb
+    yl = limbs(y, n)
+    ax = to_int(wave_product(n + vb, yl, limbs(V, vb)))
+    q = (ax >> sb) & ((1 << (64 * qlen)) - 1)
+    p = wave_product(qlen + nw, limbs(q, qlen), limbs(W, nw))
+    m = max(n, qlen + nw) + 1
+    r, f0 = wave_add(yl, p, True, 1, m)
+    r = r[:nw + 1]
+    t, f = wave_add(r, limbs(W, nw), True, 1, nw + 1)
+    r2 = t if f else r
+    q2, _ = wave_add(limbs(q, qlen), [], False, f, qlen)
+    _, ge = wave_add(r2, limbs(W, nw), True, 1, nw + 1)
+    return to_int(r2[:nw]), to_int(q2), (f0, f, ge)
+
+
+@pytest.mark.parametrize("i", [1, 2, 3, 5])
+def test_dec_split_vs_divmod(i):
+    """Every division of a level: q2 = y div W, r2 = y mod W, flags (1, f, 0),
+    with V = floor(2^(2 bits W) / W), y below W^2 (the level's bound) and
+    bit lengths from the level's bound (as the host plans them)."""
+    W = 10 ** (19 * 2 ** i)
+    wb = W.bit_length()
+    V = (1 << (2 * wb)) // W
+    rng = random.Random(i)
+    ys = [0, 1, W - 1, W, W + 1, W * W - 1, W * (W - 1), (1 << (2 * wb - 1)) % (W * W)]
+    ys += [rng.randrange(W * W) for _ in range(12)]
+    for y in ys:
+        ybits = 2 * wb   # the bound the level plans with (y < W^2 <= 2^(2 wb))
+        r2, q2, fl = dec_split(y, ybits, W, V, wb)
+        assert (q2, r2) == divmod(y, W), y
+        assert fl[0] == 1 and fl[2] == 0, (y, fl)
+
+
+def test_dec_split_catches_a_short_reciprocal():
+    """A reciprocal one unit short for a large quotient leaves q two below:
+    the check flag (r2 >= W) reports it."""
+    i = 3
+    W = 10 ** (19 * 2 ** i)
+    wb = W.bit_length()
+    V = (1 << (2 * wb)) // W - 2 ** 40
+    y = W * W - 1
+    r2, q2, fl = dec_split(y, 2 * wb, W, V, wb)
+    assert fl[2] == 1 and (q2, r2) != divmod(y, W)
