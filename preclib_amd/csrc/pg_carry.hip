@@ -603,6 +603,14 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C
     if (cbase >= (long)N.out_len && !(N.flag && (long)N.m >= cbase && (long)N.m < cbase + CARRY_CHUNK)) return;
     if (!fix_carry_in(C, chunk)) return;
     const int lane = threadIdx.x & 63;
+    if (v & 2) {
+        // every limb of the chunk propagates (t = 2^64 - 1; limb m, whose p is
+        // 0, is not among them): with the carry in, each becomes 0 - stored
+        // without reading the limbs again
+        const long lim = cbase + CARRY_CHUNK < (long)N.out_len ? cbase + CARRY_CHUNK : (long)N.out_len;
+        for (long j = cbase + lane; j < lim; j += 64) gptr_mut(N.out)[j] = 0;
+        return;
+    }
     for (int grp = 0; grp < CARRY_CHUNK / 64; ++grp) {
         const long j = cbase + 64 * grp + lane;
         uint64_t t[1];

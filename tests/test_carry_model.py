@@ -73,6 +73,11 @@ def normalize(slo, shi, out_len, flag=False):
             continue   # stores nothing (k_carry_fix returns at once)
         if not carry_in(c):
             continue
+        if agg[c][1]:   # every limb propagates: each becomes 0, stored without reading the limbs
+            assert not (flag and base <= m < base + CHUNK)
+            for j in range(base, min(base + CHUNK, out_len)):
+                out[j] = 0
+            continue
         for j in range(base, base + CHUNK):
             t, _, p = tgp(j)
             if j < out_len:
