@@ -554,22 +554,26 @@ int chud_split(size_t nterms, DBuf &Pout, size_t &lp, DBuf &Qout, size_t &lq, DB
             if ((st = b.run())) return st;
         }
         if ((st = lin_run(sums.data(), sums.size(), "chud_sum"))) return st;
-        std::vector<LenReq> req;
+        // the next level is planned on the length bounds (a product of la and lb
+        // limbs has la + lb; T one more), not on trimmed lengths read back, so
+        // the host plans it while this one runs: the operands only carry a few
+        // zero top limbs more
         for (size_t k = 0; k < np; ++k) {
-            req.push_back({up[k].P, (uint32_t)bound[3 * k]});
-            req.push_back({up[k].Q, (uint32_t)bound[3 * k + 1]});
-            req.push_back({up[k].T, (uint32_t)bound[3 * k + 2]});
-        }
-        std::vector<uint32_t> got;
-        if ((st = lengths(req, got))) return st;
-        for (size_t k = 0; k < np; ++k) {
-            up[k].lp = up[k].P ? got[3 * k] : 0;
-            up[k].lq = got[3 * k + 1];
-            up[k].lt = got[3 * k + 2];
+            up[k].lp = up[k].P ? bound[3 * k] : 0;
+            up[k].lq = bound[3 * k + 1];
+            up[k].lt = bound[3 * k + 2];
         }
         nodes.swap(up);
     }
-    const Node &root = nodes[0];
+    Node root = nodes[0];
+    {   // the root's trimmed lengths
+        std::vector<LenReq> req{{root.P, (uint32_t)root.lp}, {root.Q, (uint32_t)root.lq}, {root.T, (uint32_t)root.lt}};
+        std::vector<uint32_t> got;
+        if ((st = lengths(req, got))) return st;
+        root.lp = root.P ? got[0] : 0;
+        root.lq = got[1];
+        root.lt = got[2];
+    }
     lp = root.lp; lq = root.lq; lt = root.lt;
     if (!Pout.reset(std::max<size_t>(lp, 1)) || !Qout.reset(std::max<size_t>(lq, 1)) ||
         !Tout.reset(std::max<size_t>(lt, 1)))
