@@ -406,8 +406,10 @@ int pg_to_decimal(char *out, size_t cap, size_t *len, const uint64_t *a, size_t 
     if (!da.d) return set_error(PG_ENOMEM, "pg_to_decimal: operand");
     if (an) HIP_TRY(hipMemcpyAsync(da.d, a, an * 8, hipMemcpyHostToDevice, C.stream));
     std::string s;
-    int st = to_decimal(da.d, an, s);
-    if (st) return st;
+    size_t n = 0;
+    int st = to_decimal(da.d, an, s, 0, out, cap, &n);   // with a buffer, straight into it
+    if (len) *len = n;
+    if (st || out) return st;
     return put_string(s, out, cap, len);
 } catch (...) {
     return pg::exception_status();
@@ -453,8 +455,11 @@ int pg_pi(char *out, size_t cap, size_t *len, size_t digits, double *phase_ms) t
     LOCK;
     NEED_INIT;
     std::string s;
-    int st = pi_digits(digits, s, phase_ms);
-    if (st) return st;
+    size_t n = 0;
+    // with a buffer, the digits are formatted straight into it
+    int st = pi_digits(digits, s, phase_ms, out, cap, &n);
+    if (len) *len = n;
+    if (st || out) return st;
     return put_string(s, out, cap, len);
 } catch (...) {
     return pg::exception_status();

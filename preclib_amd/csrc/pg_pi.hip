@@ -847,15 +847,37 @@ namespace {
 size_t bits_of_top(uint64_t top, size_t limbs) { return limbs ? (limbs - 1) * 64 + 64 - (size_t)__builtin_clzll(top) : 0; }
 }  // namespace
 
-int to_decimal(const uint64_t *x, size_t xn, std::string &out, size_t point) {
+int to_decimal(const uint64_t *x, size_t xn, std::string &out, size_t point, char *dst, size_t cap, size_t *dlen) {
     Context &C = ctx();
     out.clear();
+    // the result, appended piece by piece to dst (up to cap) or to out
+    size_t pos = 0;
+    bool over = false;
+    auto put = [&](const char *p, size_t n) {
+        if (!dst) out.append(p, n);
+        else if (pos + n > cap) over = true;
+        else std::memcpy(dst + pos, p, n);
+        pos += n;
+    };
+    auto fill = [&](char ch, size_t n) {
+        if (!dst) out.append(n, ch);
+        else if (pos + n > cap) over = true;
+        else std::memset(dst + pos, ch, n);
+        pos += n;
+    };
+    auto done = [&]() {
+        if (dlen) *dlen = pos;
+        return over ? set_error(PG_EINVAL, "output capacity too small") : PG_OK;
+    };
     int st;
     size_t xb = 0;
     if ((st = top_len(x, xn, &xn))) return st;
     if (!xn) {
-        if (point) out = "0." + std::string(point, '0');
-        return PG_OK;
+        if (point) {
+            put("0.", 2);
+            fill('0', point);
+        }
+        return done();
     }
     if ((st = bit_length(x, xn, &xb))) return st;
     // chunks of 19 digits: 2^L >= ceil(digits / 19) with digits <= bits * log10(2) + 1
@@ -1075,28 +1097,27 @@ int to_decimal(const uint64_t *x, size_t xn, std::string &out, size_t point) {
     const char *sig = p + first;
     size_t n = dcap - first;
     if (!point) {
-        out.assign(sig, n);
+        if (!dst) out.reserve(n);
+        put(sig, n);
     } else {
         if (!n) {   // zero: "0" before the padding
             sig = "0";
             n = 1;
         }
         const size_t total = std::max(n, point + 1), zeros = total - n;
-        out.clear();
-        out.reserve(total + 1);
+        if (!dst) out.reserve(total + 1);
         if (zeros) {   // integer part "0", then zeros - 1 zeros and the digits
-            out.push_back('0');
-            out.push_back('.');
-            out.append(zeros - 1, '0');
-            out.append(sig, n);
+            put("0.", 2);
+            fill('0', zeros - 1);
+            put(sig, n);
         } else {
-            out.append(sig, n - point);
-            out.push_back('.');
-            out.append(sig + (n - point), point);
+            put(sig, n - point);
+            put(".", 1);
+            put(sig + (n - point), point);
         }
     }
     C.hold(std::move(img));
-    return PG_OK;
+    return done();
 }
 
 // ---- decimal parse (precn_t(std::string), src/init_basic.cpp:7-45, 66-92) -------------
@@ -1240,7 +1261,7 @@ int from_decimal(const char *digits, size_t n, std::vector<uint64_t> &out) {
 
 // ---- the pi program ------------------------------------------------------------------
 
-int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {
+int pi_digits(size_t digits, std::string &out, double phase_ms[4], char *dst, size_t cap, size_t *dlen) {
     out.clear();
     int st;
     auto now = [] { return std::chrono::steady_clock::now(); };
@@ -1309,7 +1330,10 @@ int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {
     }
     if ((st = ctx().sync())) return st;
     auto t3 = now();
-    if ((st = to_decimal(x.d, xl, out, digits))) return st;
+    size_t n = 0;
+    st = to_decimal(x.d, xl, out, digits, dst, cap, &n);
+    if (dlen) *dlen = n;
+    if (st) return st;
     auto t4 = now();
     if (phase_ms) {
         phase_ms[0] = ms(t0, t1);
@@ -1317,7 +1341,17 @@ int pi_digits(size_t digits, std::string &out, double phase_ms[4]) {
         phase_ms[2] = ms(t2, t3);
         phase_ms[3] = ms(t3, t4);
     }
-    if (out.empty()) out = "0";   // to_dec, pi_chudnovsky.cpp:154 (digits == 0)
+    if (!n) {   // to_dec, pi_chudnovsky.cpp:154 (digits == 0 and a zero value)
+        if (!dst) {
+            out = "0";
+        } else if (cap) {
+            dst[0] = '0';
+        } else {
+            if (dlen) *dlen = 1;
+            return set_error(PG_EINVAL, "output capacity too small");
+        }
+        if (dlen) *dlen = 1;
+    }
     return PG_OK;
 }
 

@@ -353,10 +353,10 @@ def to_decimal(a):
         init()
     a = _trimmed(a)
     cap = int(len(a) * 19.3) + 2
-    buf = ctypes.create_string_buffer(cap)
+    buf = np.empty(cap, np.uint8)   # (not zero-filled; the library writes the n bytes)
     n = SZ(0)
-    _check(_lib.pg_to_decimal(buf, cap, ctypes.byref(n), _ptr(a), len(a)))
-    return ctypes.string_at(buf, n.value).decode("ascii")   # the n bytes only (buf.raw copies the whole buffer)
+    _check(_lib.pg_to_decimal(buf.ctypes.data_as(ctypes.c_char_p), cap, ctypes.byref(n), _ptr(a), len(a)))
+    return str(memoryview(buf)[: n.value], "ascii")   # decoded in place, one copy
 
 
 def chud_split(nterms, need_p=True):
@@ -376,11 +376,11 @@ def pi(digits, want_phases=False):
     if not _inited:
         init()
     cap = digits + 64
-    buf = ctypes.create_string_buffer(cap)
+    buf = np.empty(cap, np.uint8)   # (not zero-filled; the library writes the n bytes)
     n = SZ(0)
     ph = (ctypes.c_double * 4)()
-    _check(_lib.pg_pi(buf, cap, ctypes.byref(n), digits, ph))
-    s = ctypes.string_at(buf, n.value).decode("ascii")
+    _check(_lib.pg_pi(buf.ctypes.data_as(ctypes.c_char_p), cap, ctypes.byref(n), digits, ph))
+    s = str(memoryview(buf)[: n.value], "ascii")   # decoded in place, one copy
     return (s, list(ph)) if want_phases else s
 
 
