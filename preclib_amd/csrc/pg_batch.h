@@ -71,6 +71,7 @@ private:
         const CarryNode *dnodes = nullptr;
         const uint32_t *dchunk = nullptr;
         uint8_t *agg = nullptr;
+        int node_kind = CARRY_ANY;   // CARRY_PLAIN: plain whole-limb nodes only
         double bytes = 0.0;   // carry traffic, for the stage timing
     };
     SumRef new_sums(size_t len);

@@ -25,7 +25,7 @@
 namespace pg {
 
 int launch_carry_nodes(size_t nnodes, double bytes, const CarryNode *dnodes, const uint32_t *dchunk,
-                       size_t nchunks, uint8_t *agg);
+                       size_t nchunks, uint8_t *agg, int node_kind);
 
 namespace {
 
@@ -360,15 +360,20 @@ int Batch::build(const pg_job *jobs, size_t n) {
     small_leaves = (SmallLeaf *)P(d_small);
     small_map = (uint32_t *)P(d_smallmap);
     small_nblocks = blk;
-    // carry nodes, grouped by depth (deepest first)
+    // carry nodes, grouped by depth (deepest first); each depth is two
+    // launches: the plain whole-limb nodes (a kernel instance with their loader
+    // alone, few registers) and the rest
     size_t chunk = 0, nidx = 0;
-    for (int d = maxdepth; d >= 0; --d) {
+    auto plain_node = [](const NodePlan &nd) { return nd.kind == NODE_PLAIN && nd.parts == 1; };
+    for (int d = maxdepth; d >= 0; --d)
+    for (int pass = 0; pass < 2; ++pass) {
         Level lv;
         lv.first_chunk = chunk;
         lv.first_node = nidx;
+        lv.node_kind = pass == 0 ? CARRY_PLAIN : CARRY_ANY;
         size_t count = 0;   // nodes of this level, written straight into the image
         for (auto &nd : nodes) {
-            if (nd.depth != d) continue;
+            if (nd.depth != d || plain_node(nd) != (pass == 0)) continue;
             CarryNode N{};
             N.kind = nd.kind;
             N.parts = nd.parts;
@@ -462,7 +467,7 @@ int Batch::enqueue() {
             return set_error(PG_EDEVICE, "pg_batch: small-product launch failed");
     }
     for (auto &lv : levels) {
-        int st = launch_carry_nodes(lv.count, lv.bytes, lv.dnodes, lv.dchunk, lv.nchunks, lv.agg);
+        int st = launch_carry_nodes(lv.count, lv.bytes, lv.dnodes, lv.dchunk, lv.nchunks, lv.agg, lv.node_kind);
         if (st) return st;
     }
     return PG_OK;

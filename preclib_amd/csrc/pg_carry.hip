@@ -454,12 +454,12 @@ __device__ __forceinline__ void limbs_t_lin(const CarryNode &N, long base, uint6
 // and the shifted-word form for linear ones (every load of a thread issued
 // before the first is used; blocked and linear nodes take limb j - 1's high
 // word from the lane below); the node is uniform across the workgroup.
-template <bool PLAIN, bool LIN, int IT>
+template <bool PLAIN, int NK, int IT>
 __device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint64_t (&t)[IT], uint32_t (&g)[IT],
                                             uint32_t (&p)[IT]) {
-    if (LIN && base - (long)(threadIdx.x & 63) >= 0) {   // a launch of linear nodes only (uniform per wave)
+    if (NK == CARRY_LIN && base - (long)(threadIdx.x & 63) >= 0) {   // linear nodes only (uniform per wave)
         limbs_t_lin<IT>(N, base, t, g, p);
-    } else if (PLAIN || (N.kind == NODE_PLAIN && N.parts == 1)) {
+    } else if (PLAIN || NK == CARRY_PLAIN || (N.kind == NODE_PLAIN && N.parts == 1)) {
         limbs_t_plain<IT>(N, base, t, g, p);
     } else if (N.kind == NODE_BLOCKED && base - (long)(threadIdx.x & 63) >= 0) {   // (uniform per wave)
         limbs_t_blocked<IT>(N, base, t, g, p);
@@ -479,7 +479,7 @@ __device__ __forceinline__ void chunk_limbs(const CarryNode &N, long base, uint6
 }
 
 // PLAIN: C.one is a plain node with whole-limb sums and there is no node table.
-template <bool PLAIN, bool LIN = false>
+template <bool PLAIN, int NK = CARRY_ANY>
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch C) {
     __shared__ Agg waves[CARRY_WAVES];
     __shared__ uint32_t wave_in[CARRY_WAVES];
@@ -494,11 +494,11 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_spec(const CarryLaunch 
     uint64_t t[IT1];
     uint32_t g[IT1], p[IT1];
     if (wave == 0 && !start) {
-        chunk_limbs<PLAIN, LIN, IT1>(N, base - 64, t, g, p);
+        chunk_limbs<PLAIN, NK, IT1>(N, base - 64, t, g, p);
     } else {
         uint64_t t0[CARRY_ITERS];
         uint32_t g0[CARRY_ITERS], p0[CARRY_ITERS];
-        chunk_limbs<PLAIN, LIN, CARRY_ITERS>(N, base, t0, g0, p0);
+        chunk_limbs<PLAIN, NK, CARRY_ITERS>(N, base, t0, g0, p0);
         t[0] = 0; g[0] = 0; p[0] = 0;
 #pragma unroll
         for (int it = 0; it < CARRY_ITERS; ++it) {
@@ -592,7 +592,7 @@ __device__ uint32_t fix_carry_in(const CarryLaunch &C, size_t chunk) {
 // whose top limbs cancel, computed for its flag over all of y but stored in
 // fewer limbs, otherwise walked back and rippled through every chunk of its
 // propagating top part.
-template <bool PLAIN, bool LIN = false>
+template <bool PLAIN, int NK = CARRY_ANY>
 __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C) {
     const size_t chunk = (size_t)blockIdx.x * CARRY_WAVES + (threadIdx.x >> 6);
     if (chunk >= C.nchunks) return;
@@ -615,7 +615,7 @@ __global__ void __launch_bounds__(CARRY_THREADS) k_carry_fix(const CarryLaunch C
         const long j = cbase + 64 * grp + lane;
         uint64_t t[1];
         uint32_t g[1], p[1];
-        chunk_limbs<PLAIN, LIN, 1>(N, j, t, g, p);
+        chunk_limbs<PLAIN, NK, 1>(N, j, t, g, p);
         const uint64_t stop = __ballot(!p[0]);
         const int k = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
         if (lane <= k) {   // k == 64: the whole group propagates
@@ -637,10 +637,14 @@ hipError_t launch_carry_kernels(const CarryLaunch &C, hipStream_t stream) {
         hipLaunchKernelGGL(k_carry_spec<true>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
         if (!one_pass)
             hipLaunchKernelGGL(k_carry_fix<true>, dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
-    } else if (C.lin_only) {   // its own instance: the linear loader's registers stay out of the others
-        hipLaunchKernelGGL((k_carry_spec<false, true>), dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
+    } else if (C.node_kind == CARRY_LIN) {   // instances of their own: each loader's registers stay
+        hipLaunchKernelGGL((k_carry_spec<false, CARRY_LIN>), dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
         if (!one_pass)
-            hipLaunchKernelGGL((k_carry_fix<false, true>), dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+            hipLaunchKernelGGL((k_carry_fix<false, CARRY_LIN>), dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
+    } else if (C.node_kind == CARRY_PLAIN) {   // out of the other node kinds' launches
+        hipLaunchKernelGGL((k_carry_spec<false, CARRY_PLAIN>), dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
+        if (!one_pass)
+            hipLaunchKernelGGL((k_carry_fix<false, CARRY_PLAIN>), dim3(nfix), dim3(CARRY_THREADS), 0, stream, C);
     } else {
         hipLaunchKernelGGL(k_carry_spec<false>, dim3(nch), dim3(CARRY_THREADS), 0, stream, C);
         if (!one_pass)

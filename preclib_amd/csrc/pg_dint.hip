@@ -98,8 +98,9 @@ int lin_run(const CarryNode *nodes, size_t n, const char *stage) {
     cl.nchunks = nch;
     cl.nnodes = n;
     cl.agg = (uint8_t *)flags.d;
-    cl.lin_only = 1;
-    for (size_t i = 0; i < n; ++i) cl.lin_only &= nodes[i].kind == NODE_LIN;
+    bool lin = true;
+    for (size_t i = 0; i < n; ++i) lin &= nodes[i].kind == NODE_LIN;
+    cl.node_kind = lin ? CARRY_LIN : CARRY_ANY;
     hipEvent_t e0 = nullptr;
     C.stage_begin(e0);
     HIP_TRY(launch_carry_kernels(cl, C.stream));

@@ -359,8 +359,9 @@ struct CarryLaunch {
     size_t nchunks;
     uint8_t *agg;               // per-chunk G | P << 1 | used carry-in << 2 | resolved << 3
     size_t nnodes;              // nodes of the launch (1 for `one`; 0: not stated)
-    int lin_only;               // every node is NODE_LIN (loaded by limbs_t_lin)
+    int node_kind;              // CARRY_ANY, or every node linear / plain whole-limb
 };
+enum { CARRY_ANY = 0, CARRY_LIN = 1, CARRY_PLAIN = 2 };
 
 // Enqueues the carry of C on `stream`: the speculative pass and the fix-up
 // pass (pg_carry.hip).  When every chunk is the first of its node

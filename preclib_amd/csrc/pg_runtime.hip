@@ -587,9 +587,10 @@ int fill_fft_tables(FftLaunch &L) {
 }
 
 int launch_carry_nodes(size_t nnodes, double bytes, const CarryNode *dnodes, const uint32_t *dchunk,
-                       size_t nchunks, uint8_t *agg) {
+                       size_t nchunks, uint8_t *agg, int node_kind) {
     Context &C = ctx();
     CarryLaunch cl{};
+    cl.node_kind = node_kind;
     cl.nodes = dnodes;
     cl.chunk_node = dchunk;
     cl.nchunks = nchunks;
